@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Headline benchmark: cuckoo trials/s at n = 2^14, k = 3, bit-exact.
+
+Workload: bgh25 insertion, n = 16384 items, m = 24576 single-slot buckets,
+k = 3, stash 0, max 500 kicks, ell_i 16, master seed 1, batches of 2^20
+trials.  A step runs the next --trials trials of that sequence (per-trial keys
+are AES-derived from (seed, batch, offset), so inputs are generated on the
+device).  Timing brackets exactly --steps steps with device synchronisation on
+both sides.  The JSON line carries the roofline of the dominant kernel (the
+insertion chains, timed with HIP events on the library's stream) and the CPU
+oracle's rate on a bounded sample of the same workload, whose failure set is
+also checked against the GPU's.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "tests"))
+
+METRIC = "cuckoo trials/sec at n=2^14,k=3 (bit-exact); sweep jobs/sec + p50/p99"
+BT = 20
+SEED = 1
+SHAPE = dict(mode="bgh25", pnum=16384, ell=16, kick=500)
+HBM_PEAK_GBS = 8000.0
+# Integer VALU peak of MI355X: 256 CUs x 64 lanes x 2.4 GHz.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# Algorithmic integer work per unit, the constants used for the int-op model:
+AES_OPS = 336          # one AES-128 block in T-table form
+KEYSCHED_OPS = 160     # one AES-128 key schedule
+SPLITMIX_OPS = 22      # one SplitMix64 draw (int32-equivalent)
+CACHED_ROUND_OPS = 4   # one cached Feistel round (lookup index, xor, mask, swap)
+EVAL_FIXED_OPS = 6     # element, y = k*e+w, cycle-walk test, division
+PROBE_OPS = 20         # choice draw (div/mod by k), table read, compare, write, advance
+SLOT_BYTES = 8         # one probe reads a 4-byte slot and writes at most one
+
+
+def int_ops_per_trial(p, probes_per_trial: float) -> float:
+    accept = (3 << p.ell_i) / p.domain
+    cache = (8 << p.half_bits) * AES_OPS
+    chain = probes_per_trial * (8 / accept * CACHED_ROUND_OPS + EVAL_FIXED_OPS) + probes_per_trial * PROBE_OPS
+    return 2 * AES_OPS + KEYSCHED_OPS + 2 * SPLITMIX_OPS + cache + chain
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ctx, ck, p, seconds: float):
+    """Oracle rate on a bounded prefix of the workload; also the parity check."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ck_oracle as ock
+
+    op = ock.parameters(SHAPE["mode"], SHAPE["pnum"], SHAPE["ell"], kick=SHAPE["kick"])
+    cores = host_cores()
+    bk = ock.batch_key(SEED, 0)
+    t0 = time.perf_counter()
+    ock.run_range(bk, 0, cores * 2, op, threads=cores)
+    est = (cores * 2) / max(time.perf_counter() - t0, 1e-6)
+    sample = int(max(cores * 4, min(est * seconds, 1 << 16)))
+    t0 = time.perf_counter()
+    of, _, ost = ock.run_range(bk, 0, sample, op, threads=cores)
+    dt = time.perf_counter() - t0
+    gf, _ = ctx.launch(p, ck.batch_key(SEED, 0), 0, sample)
+    parity = list(gf) == list(of)
+    return {
+        "value": sample / dt, "unit": "trials/s", "cores": cores, "kind": "port",
+        "sample": f"first {sample} trials of batch 0 (seed 1), oracle/ck_oracle.c with AES-NI round cache, "
+                  f"{cores} threads on {cpu_model()}",
+        "seconds": dt, "failures": len(of), "gpu_failures_equal": parity,
+    }, parity
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--trials", type=int, default=1 << 17, help="trials per step (power of two <= 2^20)")
+    ap.add_argument("--tier", choices=["auto", "hbm", "lds"], default="auto")
+    ap.add_argument("--chunk", type=int, default=0, help="trials per device pass (0 = library default)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(trial batches would shard across GPUs by batch id; see DESIGN.md)", file=sys.stderr)
+        return 2
+    t = args.trials
+    if t <= 0 or t & (t - 1) or t > (1 << BT):
+        print("--trials must be a power of two <= 2^20", file=sys.stderr)
+        return 2
+
+    import torch
+    from paths import load_product
+
+    ck = load_product()
+    p = ck.params(SHAPE["mode"], SHAPE["pnum"], SHAPE["ell"], kick=SHAPE["kick"])
+    ctx = ck.Context(0)
+    ctx.self_test()
+    ctx.set_tier({"auto": ck.TIER_AUTO, "hbm": ck.TIER_HBM, "lds": ck.TIER_LDS}[args.tier])
+    if args.chunk:
+        ctx.set_chunk(args.chunk)
+
+    keys = {}
+
+    def step(i):
+        g = i * t
+        batch, first = g >> BT, g & ((1 << BT) - 1)
+        if batch not in keys:
+            keys[batch] = ck.batch_key(SEED, batch)
+        return ctx.launch(p, keys[batch], first, t)
+
+    for i in range(args.warmup):
+        step(i)
+        print(f"warmup step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    agg = dict(failed=0, probes=0, build_ms=0.0, chain_ms=0.0, launches=0, chunk=0, tier=0)
+    for i in range(args.warmup, args.warmup + args.steps):
+        f, st = step(i)
+        agg["failed"] += len(f)
+        agg["probes"] += st.probes
+        agg["build_ms"] += st.build_ms
+        agg["chain_ms"] += st.chain_ms
+        agg["launches"] += st.launches
+        agg["chunk"], agg["tier"] = st.chunk, st.tier
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    trials = args.steps * t
+    value = trials / elapsed
+    chain_launches = max(1, agg["launches"] // 3)
+    chain_avg_s = agg["chain_ms"] / 1e3 / chain_launches
+    bytes_per_launch = SLOT_BYTES * agg["probes"] / chain_launches
+    achieved = bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else 0.0
+    probes_per_trial = agg["probes"] / trials
+    ops = int_ops_per_trial(p, probes_per_trial)
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "trials/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic: per-trial keys AES-derived on device from (seed 1, batch, offset)",
+        "config": {"workload": "bgh25 n=16384 m=24576 k=3 stash=0 kick=500 ell_i=16 seed=1 BT=20 sequential",
+                   "trials_per_step": t, "tier": {1: "hbm", 2: "lds"}.get(agg["tier"], "auto"),
+                   "trials_per_pass": agg["chunk"]},
+        "insertions_per_s": value * p.pnum,
+        "probes_per_s": agg["probes"] / elapsed,
+        "probes_per_trial": probes_per_trial,
+        "failed": agg["failed"],
+        "roofline": {
+            "kernel": "k_chain<3,bgh> (insertion chains, one lane per trial)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_ms": chain_avg_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "int_ops_model": {
+            "ops_per_trial": ops, "achieved_tops": ops * value / 1e12, "peak_tops": VALU_PEAK_TOPS,
+            "frac": ops * value / 1e12 / VALU_PEAK_TOPS,
+        },
+        "device_ms": {"build": agg["build_ms"], "chain": agg["chain_ms"]},
+    }
+    ok = True
+    if not args.no_cpu:
+        rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)
+    ctx.close()
+    print(json.dumps(rec), flush=True)
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

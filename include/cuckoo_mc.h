@@ -1,0 +1,119 @@
+/*
+ * cuckoo_mc.h -- C ABI of the MI355X cuckoo-hashing Monte Carlo library
+ * (pbc-based-dmpf-monte-carlo-test_amd/lib/libcuckoo_mc.so).
+ *
+ * The library replaces the reference's device half: the launch callable that
+ * process_batch() drives (cuckoo_bench.cu:724-745, :979-1034), the AES/PRP
+ * kernels (cuckoo_bench.cu:102-317, gpu_cached.cuh, gpu_shared.cuh) and their
+ * known-answer self tests (cuckoo_bench.cu:322-354).  Plain pointers and sizes
+ * only; no exceptions cross the ABI: every call returns CK_OK or a negative
+ * code, and ck_last_error() describes the last failure on the calling thread.
+ *
+ * Threading: one ck_ctx per host thread (the reference binds one worker
+ * thread per GPU, cuckoo_bench.cu:1102-1118).  Calls on one ctx are
+ * synchronous unless named *_async.
+ */
+#ifndef CUCKOO_MC_H
+#define CUCKOO_MC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CK_OK = 0,
+    CK_EINVAL = -1,   /* invalid argument or shape */
+    CK_EDEVICE = -2,  /* HIP runtime error */
+    CK_ESELFTEST = -3,/* AES known-answer test failed */
+    CK_ERANGE = -4,   /* device counter out of range (cuckoo_bench.cu:1027) */
+    CK_ENOMEM = -5
+};
+
+enum { CK_MODE_BGH25 = 1, CK_MODE_DRRT18 = 2 }; /* journal encoding, cuckoo_bench.cu:607 */
+#define CK_DERIVED 0xffffffffu
+
+/* Experiment shape.  Mirrors Options+Params of the reference
+ * (cuckoo_bench.cu:406-419, :499-526) plus the extension: k candidate buckets
+ * (reference: 3), explicit buckets/kick (reference: derived from mode), stash
+ * slots (reference: 0). */
+typedef struct ck_params {
+    uint32_t mode, k, pnum, buckets, kick, stash, ell_i, input_mode;
+    uint32_t half_bits, ell_d;
+    uint64_t bucket_size, domain, mask, range_mask;
+} ck_params;
+
+/* Table placement tier for the insertion chains. */
+enum {
+    CK_TIER_AUTO = 0,
+    CK_TIER_HBM = 1,  /* one lane per trial, table and bucket map in HBM */
+    CK_TIER_LDS = 2   /* trials per workgroup, table and bucket map in LDS */
+};
+
+typedef struct ck_stats {
+    uint64_t trials;      /* trials completed by the call */
+    uint64_t failed;      /* failing trials */
+    uint64_t probes;      /* table-slot reads of the insertion chains */
+    uint32_t tier;        /* tier actually used */
+    uint32_t chunk;       /* trials per device pass */
+    double build_ms;      /* device time of the key/round-cache/map kernels */
+    double chain_ms;      /* device time of the insertion kernels */
+    double fused_ms;      /* device time of fused (LDS tier) kernels */
+    uint32_t launches;    /* kernel launches */
+    uint32_t pad;
+} ck_stats;
+
+typedef struct ck_ctx ck_ctx;
+
+/* parameters() (cuckoo_bench.cu:500-526) plus the extension.  Pass
+ * CK_DERIVED as buckets/kick for the reference's mode-derived values. */
+int ck_params_derive(uint32_t mode, uint32_t pnum, uint32_t ell_i, uint32_t k, uint32_t buckets,
+                     uint32_t kick, uint32_t stash, uint32_t input_mode, ck_params* out);
+
+/* batch_key(seed, batch_id) (cuckoo_bench.cu:393-404): 44 round-key words. */
+void ck_batch_key(uint64_t seed, uint64_t batch_id, uint32_t round_keys[44]);
+
+/* Host AES-128 on four big-endian words (host_aes128, cuckoo_bench.cu:360-392). */
+void ck_host_aes(const uint32_t round_keys[44], const uint32_t in[4], uint32_t out[4]);
+
+/* One context per host thread and device (gpu_worker, cuckoo_bench.cu:813-973).
+ * Uploads the AES tables to the device (upload_aes_tables, :64-75). */
+int ck_ctx_create(int device, ck_ctx** out);
+void ck_ctx_destroy(ck_ctx* ctx);
+
+/* Forces a tier (CK_TIER_*) for later launches; CK_TIER_AUTO picks by shape. */
+int ck_ctx_set_tier(ck_ctx* ctx, uint32_t tier);
+/* Caps the trials per device pass (0 = library default). */
+int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
+
+/* FIPS-197 AES-128 known answer through the device AES path
+ * (aes_self_test / cached_aes_self_test, cuckoo_bench.cu:332-354). */
+int ck_aes_self_test(ck_ctx* ctx);
+
+/* The launch callable of process_batch (cuckoo_bench.cu:980-1031): runs trials
+ * [first, first+count) of the batch whose key schedule is round_keys.
+ * failed_offsets (capacity >= count) receives the failing offsets in
+ * ascending order, *n_failed their number.  stats may be NULL. */
+int ck_launch(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t first,
+              uint32_t count, uint32_t* failed_offsets, uint32_t* n_failed, ck_stats* stats);
+
+/* Test hook: like ck_launch, but also returns per-trial probe counts
+ * (probes[i] for trial first+i) and, when tables is non-NULL and the shape
+ * allows, the final table of every trial (tables[i*buckets + b], entry =
+ * ordinal | choice<<30, empty = 0xffffffff). */
+int ck_launch_debug(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t first,
+                    uint32_t count, uint32_t* failed_offsets, uint32_t* n_failed, uint32_t* probes,
+                    uint32_t* tables, ck_stats* stats);
+
+/* Test hook: the device key derivation and bucket map of one trial:
+ * words[0..43] = PRP key schedule, walk/randomness of the first BGH draw,
+ * map[o*k+w] = bucket(o, w). */
+int ck_trial_map(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t offset,
+                 uint32_t key_words[44], uint64_t* walk, uint64_t* randomness, uint32_t* map);
+
+const char* ck_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

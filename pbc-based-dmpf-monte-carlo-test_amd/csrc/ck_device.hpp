@@ -1,0 +1,230 @@
+// ck_device.hpp -- device building blocks shared by the HIP kernels:
+// AES-128 over an LDS T-table, the AES key schedule, SplitMix64, the Feistel
+// PRP bucket function (round-cache and direct forms), exact division by the
+// bucket size, and the ballot/popcount failure append.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ck {
+
+constexpr uint32_t EMPTY = 0xffffffffu;
+constexpr uint32_t ORD_MASK = 0x3fffffffu;  // table entry = ordinal | choice << 30
+constexpr int TE_REPL = 32;                 // copies of T0 in LDS: lane L reads copy L % 32
+
+// Kernel-argument form of ck_params plus the host-precomputed divider.
+struct Shape {
+    uint32_t k, pnum, buckets, kick, stash, half, input_mode, mode;
+    uint64_t bucket_size, domain, mask, range_mask;
+    uint64_t div_magic;  // ceil(2^64 / bucket_size) for the 32-bit quotient; 0 when bucket_size == 1
+};
+struct RoundKeys {
+    uint32_t w[44];
+};
+
+__device__ __forceinline__ uint32_t rotr(uint32_t x, uint32_t s) { return __builtin_amdgcn_alignbit(x, x, s); }
+
+// T0 replicated TE_REPL times: entry x of copy c sits at x*TE_REPL + c, so a
+// wave whose lane L reads copy L%32 touches bank L%32 whatever the byte index.
+struct LdsTab {
+    const uint32_t* tab;
+    uint32_t lane;  // lane % TE_REPL
+    __device__ __forceinline__ uint32_t t(uint32_t x) const { return tab[(x << 5) | lane]; }
+    __device__ __forceinline__ uint32_t s(uint32_t x) const { return (t(x) >> 16) & 255u; }
+};
+
+// FIPS-197 AES-128 encryption in the T-table form (Daemen & Rijmen): rounds
+// 1..9 are T0[b0]^ror8(T0[b1])^ror16(T0[b2])^ror24(T0[b3])^rk, the last round
+// substitutes bytes only.  Input and output are big-endian column words.
+__device__ __forceinline__ void aes_enc(const LdsTab& T, const uint32_t* rk, uint32_t a, uint32_t b, uint32_t c,
+                                        uint32_t d, uint32_t out[4]) {
+    uint32_t s0 = a ^ rk[0], s1 = b ^ rk[1], s2 = c ^ rk[2], s3 = d ^ rk[3];
+#pragma unroll
+    for (int r = 1; r < 10; ++r) {
+        const uint32_t n0 = T.t(s0 >> 24) ^ rotr(T.t((s1 >> 16) & 255), 8) ^ rotr(T.t((s2 >> 8) & 255), 16) ^
+                            rotr(T.t(s3 & 255), 24) ^ rk[4 * r];
+        const uint32_t n1 = T.t(s1 >> 24) ^ rotr(T.t((s2 >> 16) & 255), 8) ^ rotr(T.t((s3 >> 8) & 255), 16) ^
+                            rotr(T.t(s0 & 255), 24) ^ rk[4 * r + 1];
+        const uint32_t n2 = T.t(s2 >> 24) ^ rotr(T.t((s3 >> 16) & 255), 8) ^ rotr(T.t((s0 >> 8) & 255), 16) ^
+                            rotr(T.t(s1 & 255), 24) ^ rk[4 * r + 2];
+        const uint32_t n3 = T.t(s3 >> 24) ^ rotr(T.t((s0 >> 16) & 255), 8) ^ rotr(T.t((s1 >> 8) & 255), 16) ^
+                            rotr(T.t(s2 & 255), 24) ^ rk[4 * r + 3];
+        s0 = n0;
+        s1 = n1;
+        s2 = n2;
+        s3 = n3;
+    }
+    out[0] = ((T.s(s0 >> 24) << 24) | (T.s((s1 >> 16) & 255) << 16) | (T.s((s2 >> 8) & 255) << 8) | T.s(s3 & 255)) ^
+             rk[40];
+    out[1] = ((T.s(s1 >> 24) << 24) | (T.s((s2 >> 16) & 255) << 16) | (T.s((s3 >> 8) & 255) << 8) | T.s(s0 & 255)) ^
+             rk[41];
+    out[2] = ((T.s(s2 >> 24) << 24) | (T.s((s3 >> 16) & 255) << 16) | (T.s((s0 >> 8) & 255) << 8) | T.s(s1 & 255)) ^
+             rk[42];
+    out[3] = ((T.s(s3 >> 24) << 24) | (T.s((s0 >> 16) & 255) << 16) | (T.s((s1 >> 8) & 255) << 8) | T.s(s2 & 255)) ^
+             rk[43];
+}
+
+// Only the last output word (the Feistel round function uses word 3).
+__device__ __forceinline__ uint32_t aes_last(const LdsTab& T, const uint32_t* rk, uint32_t a, uint32_t b, uint32_t c,
+                                             uint32_t d) {
+    uint32_t o[4];
+    aes_enc(T, rk, a, b, c, d, o);
+    return o[3];
+}
+
+// FIPS-197 5.2 key expansion from four big-endian words (the device derives
+// each trial's PRP key this way, cuckoo_bench.cu:136-149).
+__device__ __forceinline__ void expand_key_dev(const LdsTab& T, const uint32_t w[4], uint32_t rk[44]) {
+    rk[0] = w[0];
+    rk[1] = w[1];
+    rk[2] = w[2];
+    rk[3] = w[3];
+    uint32_t rcon = 1;
+#pragma unroll
+    for (int i = 4; i < 44; ++i) {
+        uint32_t v = rk[i - 1];
+        if ((i & 3) == 0) {
+            v = ((T.s((v >> 16) & 255) << 24) | (T.s((v >> 8) & 255) << 16) | (T.s(v & 255) << 8) | T.s(v >> 24)) ^
+                (rcon << 24);
+            rcon = ((rcon << 1) ^ ((rcon & 0x80) ? 0x1b : 0)) & 0xff;
+        }
+        rk[i] = rk[i - 4] ^ v;
+    }
+}
+
+// SplitMix64 (Steele, Lea & Flood), the walk generator (cuckoo_core.hpp:110-114).
+__device__ __forceinline__ uint64_t splitmix(uint64_t& state) {
+    state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Exact y / d for 32-bit y with M = ceil(2^64/d) (Lemire, Kaser & Kurz,
+// "Faster remainder by direct computation", 2019): floor(M*y / 2^64).
+__device__ __forceinline__ uint32_t div_bucket(const Shape& S, uint32_t y) {
+    if (S.div_magic == 0) return y;  // bucket_size == 1
+    const uint32_t lo = static_cast<uint32_t>(S.div_magic), hi = static_cast<uint32_t>(S.div_magic >> 32);
+    const uint64_t mid = uint64_t(hi) * y + __umulhi(lo, y);
+    return static_cast<uint32_t>(mid >> 32);
+}
+
+// Per-trial derivation (cuckoo_bench.cu:222-238): PRP key = schedule of
+// AES_batch("KEY1",0,offset,"PRP1"); (walk, affine shift) from
+// AES_batch("WALK",0,offset,"RNG1"); step and the first walk draw by SplitMix64.
+struct TrialSeed {
+    uint32_t rk[44];
+    uint64_t walk, rnd, shift, step;
+};
+__device__ __forceinline__ void derive_trial(const LdsTab& T, const RoundKeys& bk, const Shape& S, uint32_t offset,
+                                             TrialSeed& ts) {
+    uint32_t w[4];
+    aes_enc(T, bk.w, 0x4b455931u, 0u, offset, 0x50525031u, w);
+    expand_key_dev(T, w, ts.rk);
+    aes_enc(T, bk.w, 0x57414c4bu, 0u, offset, 0x524e4731u, w);
+    ts.walk = (uint64_t(w[0]) << 32) | w[1];
+    ts.shift = ((uint64_t(w[2]) << 32) | w[3]) & S.range_mask;
+    uint64_t input_state = ts.shift ^ 0xd1b54a32d192ed03ull;
+    ts.step = (splitmix(input_state) | 1ull) & S.range_mask;
+    ts.rnd = splitmix(ts.walk);
+}
+
+__device__ __forceinline__ uint64_t element_of(const Shape& S, uint64_t shift, uint64_t step, uint32_t ordinal) {
+    return S.input_mode == 0 ? uint64_t(ordinal) : (shift + uint64_t(ordinal) * step) & S.range_mask;
+}
+
+// Feistel PRP over the round cache (half <= 16, so y and the domain fit 32 bits):
+// 8 rounds L,R <- R, L ^ F_r(R), cycle-walked into [0, domain), then / bucket_size.
+__device__ __forceinline__ uint32_t feistel_cached(const uint16_t* cache, const Shape& S, uint64_t elem, uint32_t k,
+                                                   uint32_t which) {
+    const uint32_t h = S.half, m = static_cast<uint32_t>(S.mask);
+    uint32_t y = static_cast<uint32_t>(elem * k + which);
+    do {
+        uint32_t L = y >> h, R = y & m;
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) {
+            const uint32_t nx = L ^ cache[(r << h) + R];
+            L = R;
+            R = nx & m;
+        }
+        y = (L << h) | R;
+    } while (uint64_t(y) >= S.domain);
+    return div_bucket(S, y);
+}
+
+// Feistel PRP evaluating AES directly (any half): the round function is the
+// last word of AES_key("CESP", round, half, right) (cuckoo_bench.cu:161-184).
+__device__ __noinline__ uint32_t feistel_direct(const LdsTab& T, const uint32_t* rk, const Shape& S, uint64_t elem,
+                                                uint32_t k, uint32_t which) {
+    uint64_t y = elem * k + which;
+    do {
+        uint64_t L = y >> S.half, R = y & S.mask;
+#pragma unroll 1
+        for (uint32_t r = 0; r < 8; ++r) {
+            const uint32_t f = aes_last(T, rk, 0x43455350u, r, S.half, static_cast<uint32_t>(R));
+            const uint64_t nx = (L ^ (uint64_t(f) & S.mask)) & S.mask;
+            L = R;
+            R = nx;
+        }
+        y = (L << S.half) | R;
+    } while (y >= S.domain);
+    if (S.domain <= 0xffffffffull) return div_bucket(S, static_cast<uint32_t>(y));
+    return static_cast<uint32_t>(y / S.bucket_size);
+}
+
+// which = r % K, r /= K for the 64-bit walk word (cuckoo_core.hpp:126-127).
+template <int K>
+__device__ __forceinline__ uint32_t take_choice(uint64_t& r) {
+    if constexpr (K == 2) {
+        const uint32_t w = static_cast<uint32_t>(r) & 1u;
+        r >>= 1;
+        return w;
+    } else if constexpr (K == 4) {
+        const uint32_t w = static_cast<uint32_t>(r) & 3u;
+        r >>= 2;
+        return w;
+    } else {
+        const uint64_t q = __umul64hi(r, 0xAAAAAAAAAAAAAAABull) >> 1;  // floor(r/3)
+        const uint32_t w = static_cast<uint32_t>(r - q * 3);
+        r = q;
+        return w;
+    }
+}
+
+// Wave-local LDS visibility: every lane's earlier LDS writes are seen by the
+// wave's later LDS reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Failure counting: one 64-bit ballot per wave, one atomic per wave with a
+// failure, each failing lane writes at base + its rank among failing lanes.
+__device__ __forceinline__ void append_failures(bool failed, uint32_t offset, uint32_t* __restrict__ count,
+                                                uint32_t* __restrict__ offsets) {
+    const uint64_t m = __ballot(failed);
+    if (m == 0) return;
+    const int leader = __builtin_ctzll(m);
+    const uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == static_cast<uint32_t>(leader)) base = atomicAdd(count, static_cast<uint32_t>(__popcll(m)));
+    base = __shfl(base, leader);
+    if (failed) {
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+        offsets[base + rank] = offset;
+    }
+}
+
+// Sum of a per-lane count over the wave, added with one 64-bit atomic.
+__device__ __forceinline__ void add_wave_total(uint32_t v, unsigned long long* __restrict__ total) {
+    uint64_t s = v;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    if (__lane_id() == 0 && s) atomicAdd(total, static_cast<unsigned long long>(s));
+}
+
+}  // namespace ck

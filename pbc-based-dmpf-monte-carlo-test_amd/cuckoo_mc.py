@@ -1,0 +1,173 @@
+"""Python view of libcuckoo_mc.so (the HIP library) through its C ABI.
+
+This is the product path for tests and bench.py: every call runs the gfx950
+kernels.  There is no CPU fallback; loading fails loudly when the library is
+not built.  The ABI is declared in include/cuckoo_mc.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libcuckoo_mc.so"
+
+CK_OK = 0
+BGH25, DRRT18 = 1, 2
+DERIVED = 0xFFFFFFFF
+TIER_AUTO, TIER_HBM, TIER_LDS = 0, 1, 2
+MODES = {"bgh25": BGH25, "drrt18": DRRT18}
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("mode", "k", "pnum", "buckets", "kick", "stash",
+                                            "ell_i", "input_mode", "half_bits", "ell_d")] + \
+               [(n, C.c_uint64) for n in ("bucket_size", "domain", "mask", "range_mask")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Stats(C.Structure):
+    _fields_ = [("trials", C.c_uint64), ("failed", C.c_uint64), ("probes", C.c_uint64),
+                ("tier", C.c_uint32), ("chunk", C.c_uint32), ("build_ms", C.c_double),
+                ("chain_ms", C.c_double), ("fused_ms", C.c_double), ("launches", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class CkError(RuntimeError):
+    pass
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise FileNotFoundError(
+            f"{LIB_PATH} is not built: run `make -C {HERE}` (or __graft_entry__.build()); "
+            "there is no CPU fallback for this path")
+    lib = C.CDLL(str(LIB_PATH))
+    P, U32, U64 = C.POINTER, C.c_uint32, C.c_uint64
+    vp = C.c_void_p
+    sig = {
+        "ck_params_derive": (C.c_int, [U32] * 8 + [P(Params)]),
+        "ck_batch_key": (None, [U64, U64, P(U32)]),
+        "ck_host_aes": (None, [P(U32), P(U32), P(U32)]),
+        "ck_ctx_create": (C.c_int, [C.c_int, P(vp)]),
+        "ck_ctx_destroy": (None, [vp]),
+        "ck_ctx_set_tier": (C.c_int, [vp, U32]),
+        "ck_ctx_set_chunk": (C.c_int, [vp, U32]),
+        "ck_aes_self_test": (C.c_int, [vp]),
+        "ck_launch": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(Stats)]),
+        "ck_launch_debug": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U32),
+                                      P(Stats)]),
+        "ck_trial_map": (C.c_int, [vp, P(Params), P(U32), U32, P(U32), P(U64), P(U64), P(U32)]),
+        "ck_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load()
+
+# Every symbol declared in include/cuckoo_mc.h.
+EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "ck_ctx_destroy",
+           "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_aes_self_test", "ck_launch", "ck_launch_debug",
+           "ck_trial_map", "ck_last_error"]
+
+
+def _check(rc):
+    if rc != CK_OK:
+        raise CkError(f"code {rc}: {LIB.ck_last_error().decode()}")
+
+
+def _u32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def params(mode, pnum, ell_i, k=3, buckets=DERIVED, kick=DERIVED, stash=0, input_mode=0) -> Params:
+    if isinstance(mode, str):
+        mode = MODES[mode]
+    p = Params()
+    _check(LIB.ck_params_derive(mode, pnum, ell_i, k, buckets, kick, stash, input_mode, C.byref(p)))
+    return p
+
+
+def batch_key(seed: int, batch: int) -> np.ndarray:
+    rk = np.zeros(44, dtype=np.uint32)
+    LIB.ck_batch_key(seed, batch, _u32p(rk))
+    return rk
+
+
+def host_aes(rk: np.ndarray, block) -> list:
+    out = (C.c_uint32 * 4)()
+    LIB.ck_host_aes(_u32p(np.ascontiguousarray(rk, dtype=np.uint32)), (C.c_uint32 * 4)(*block), out)
+    return list(out)
+
+
+class Context:
+    """One device context (one host thread), as gpu_worker holds per GPU."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(LIB.ck_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            LIB.ck_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tier(self, tier: int):
+        _check(LIB.ck_ctx_set_tier(self.h, tier))
+
+    def set_chunk(self, trials: int):
+        _check(LIB.ck_ctx_set_chunk(self.h, trials))
+
+    def self_test(self):
+        _check(LIB.ck_aes_self_test(self.h))
+
+    def launch(self, p: Params, rk: np.ndarray, first: int, count: int):
+        """Failing offsets (sorted) of trials [first, first+count) and stats."""
+        rk = np.ascontiguousarray(rk, dtype=np.uint32)
+        out = np.zeros(max(count, 1), dtype=np.uint32)
+        n = C.c_uint32()
+        st = Stats()
+        _check(LIB.ck_launch(self.h, C.byref(p), _u32p(rk), first, count, _u32p(out), C.byref(n), C.byref(st)))
+        return out[:n.value].copy(), st
+
+    def launch_debug(self, p: Params, rk: np.ndarray, first: int, count: int, tables=False):
+        rk = np.ascontiguousarray(rk, dtype=np.uint32)
+        out = np.zeros(max(count, 1), dtype=np.uint32)
+        probes = np.zeros(max(count, 1), dtype=np.uint32)
+        tab = np.zeros(count * p.buckets, dtype=np.uint32) if tables else None
+        n = C.c_uint32()
+        st = Stats()
+        _check(LIB.ck_launch_debug(self.h, C.byref(p), _u32p(rk), first, count, _u32p(out), C.byref(n),
+                                   _u32p(probes), _u32p(tab) if tab is not None else None, C.byref(st)))
+        return (out[:n.value].copy(), probes[:count].copy(),
+                tab.reshape(count, p.buckets) if tab is not None else None, st)
+
+    def trial_map(self, p: Params, rk: np.ndarray, offset: int):
+        rk = np.ascontiguousarray(rk, dtype=np.uint32)
+        key = np.zeros(44, dtype=np.uint32)
+        walk, rnd = C.c_uint64(), C.c_uint64()
+        m = np.zeros(p.pnum * p.k, dtype=np.uint32)
+        _check(LIB.ck_trial_map(self.h, C.byref(p), _u32p(rk), offset, _u32p(key), C.byref(walk),
+                                C.byref(rnd), _u32p(m)))
+        return key, walk.value, rnd.value, m

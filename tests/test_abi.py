@@ -1,0 +1,74 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol,
+and its host-side pieces (shapes, batch keys, host AES) agree with the oracle.
+No device calls."""
+import ctypes as C
+import random
+import re
+
+import numpy as np
+import pytest
+
+import ck_oracle as ock
+from paths import HEADER, load_product
+
+ck = load_product()
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(ck_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_symbols()
+    assert names, "no declarations parsed"
+    for n in names:
+        assert hasattr(ck.LIB, n), n
+    assert sorted(ck.EXPORTS) == names
+
+
+def test_params_match_oracle():
+    rng = random.Random(4)
+    for _ in range(300):
+        mode = rng.choice([1, 2])
+        k = rng.choice([2, 3, 4])
+        pnum = rng.choice([3, 16, 100, 1024, 16384, 65534, 1 << 20])
+        ell = max(rng.choice([8, 16, 20, 32, 61]), (pnum - 1).bit_length())
+        b = rng.choice([ck.DERIVED, int(pnum * rng.uniform(1.2, 1.6))])
+        kick = rng.choice([ck.DERIVED, 0, 100, 500, 1000])
+        stash = rng.choice([0, 1, 4])
+        im = rng.choice([0, 1])
+        a = ck.params(mode, pnum, ell, k=k, buckets=b, kick=kick, stash=stash, input_mode=im).as_dict()
+        o = ock.parameters(mode, pnum, ell, k=k, buckets=b, kick=kick, stash=stash, input_mode=im).as_dict()
+        assert a == o
+
+
+def test_invalid_params_rejected():
+    with pytest.raises(ck.CkError):
+        ck.params("bgh25", 16, 8, k=5)
+    with pytest.raises(ck.CkError):
+        ck.params("bgh25", 1000, 8)  # 2^8 < pnum
+    with pytest.raises(ck.CkError):
+        ck.params(7, 16, 8)
+
+
+def test_batch_key_and_host_aes(fixtures):
+    rng = random.Random(8)
+    for _ in range(50):
+        seed, batch = rng.getrandbits(64), rng.getrandbits(64)
+        assert list(ck.batch_key(seed, batch)) == ock.batch_key(seed, batch).words()
+    v = fixtures["batch_key_vectors"]
+    for case in v["cases"]:
+        assert ck.host_aes(ck.batch_key(v["seed"], case["batch"]), [0, 0, 0, 0]) == case["aes_zero"]
+    f = fixtures["fips197"]
+    rk = np.array(ock.expand_key(f["key_lo"], f["key_hi"]).words(), dtype=np.uint32)
+    assert ck.host_aes(rk, f["plain"]) == f["cipher"]
+
+
+def test_ctx_create_without_device_fails_cleanly():
+    import paths
+    if paths.gpu_available():
+        pytest.skip("a device is present")
+    h = C.c_void_p()
+    rc = ck.LIB.ck_ctx_create(0, C.byref(h))
+    assert rc != 0 and not h.value

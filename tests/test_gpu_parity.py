@@ -1,0 +1,133 @@
+"""Parity of the HIP path with the oracle, through the C ABI.
+
+Bit-exact: failure offsets, per-trial probe counts, final tables, the device
+key derivation and bucket map.  Sizes are chosen so the oracle finishes in
+seconds."""
+import numpy as np
+import pytest
+
+import ck_oracle as ock
+from paths import load_product
+
+pytestmark = pytest.mark.gpu
+
+ck = load_product()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ck.Context(0)
+    yield c
+    c.close()
+
+
+def both(mode, pnum, ell, **kw):
+    return ck.params(mode, pnum, ell, **kw), ock.parameters(mode, pnum, ell, **kw)
+
+
+def test_device_aes_self_test(ctx):
+    ctx.self_test()
+
+
+SHAPES = [
+    ("bgh25", 16, 8, {}),
+    ("drrt18", 16, 8, {"input_mode": 1}),
+    ("bgh25", 1024, 16, {"kick": 500}),
+    ("drrt18", 4096, 16, {"input_mode": 1}),
+    ("bgh25", 16384, 16, {"kick": 500}),
+    ("bgh25", 16, 32, {}),                      # half 17: direct-AES build
+    ("drrt18", 300, 24, {"input_mode": 1}),     # half 13: direct-AES build
+    ("bgh25", 200, 12, {"k": 2, "buckets": 260, "stash": 1}),
+    ("drrt18", 200, 12, {"k": 4, "buckets": 230, "stash": 2}),
+]
+
+
+@pytest.mark.parametrize("mode,pnum,ell,kw", SHAPES)
+def test_trial_derivation_and_bucket_map(ctx, mode, pnum, ell, kw):
+    p, op = both(mode, pnum, ell, **kw)
+    for seed, batch, off in [(1, 0, 0), (123, 7, 255), (2**63 + 5, 2**40 + 3, 123456)]:
+        rk = ck.batch_key(seed, batch)
+        key, walk, rnd, m = ctx.trial_map(p, rk, off)
+        bk = ock.batch_key(seed, batch)
+        t = ock.trial_setup(bk, off, op)
+        assert list(key) == t.key.words()
+        assert (walk, rnd) == (t.walk, t.randomness)
+        assert (m == ock.bucket_map(bk, off, op)).all()
+
+
+@pytest.mark.parametrize("mode", ["bgh25", "drrt18"])
+def test_golden_32768(ctx, fixtures, mode):
+    g = fixtures["golden_32768"]
+    p = ck.params(mode, g["pnum"], g["ell_i"])
+    fails = []
+    for batch in range(1 << (g["et"] - g["bt"])):
+        f, _ = ctx.launch(p, ck.batch_key(g["seed"], batch), 0, 1 << g["bt"])
+        fails += [batch * (1 << g["bt"]) + int(x) for x in f]
+    assert fails == g["failures"][mode]
+
+
+def test_affine_vectors(ctx, fixtures):
+    v = fixtures["affine_vectors"]
+    for mode, want in v["failures"].items():
+        p = ck.params(mode, v["pnum"], v["ell_i"], input_mode=1)
+        f, _ = ctx.launch(p, ck.batch_key(v["seed"], v["batch"]), 0, v["count"])
+        assert list(f) == want
+
+
+def test_reference_ranges(ctx, fixtures):
+    for row in fixtures["reference_ranges"]:
+        p = ck.params(row["mode"], row["pnum"], row["ell_i"], input_mode=row["input_mode"])
+        f, _ = ctx.launch(p, ck.batch_key(row["seed"], row["batch"]), 0, row["count"])
+        assert list(f) == row["failures"], row
+
+
+PROBE_SHAPES = [
+    ("bgh25", 16, 8, {}, 2048),
+    ("drrt18", 16, 8, {}, 2048),
+    ("bgh25", 64, 12, {"input_mode": 1, "kick": 20}, 2048),
+    ("drrt18", 1024, 16, {"kick": 3, "buckets": 1100}, 512),
+    ("bgh25", 1024, 16, {"kick": 500}, 512),
+    ("bgh25", 300, 12, {"k": 2, "buckets": 420, "stash": 2, "kick": 30}, 512),
+    ("bgh25", 300, 12, {"k": 4, "buckets": 320, "stash": 1, "kick": 10}, 512),
+    ("drrt18", 300, 12, {"k": 2, "buckets": 420, "stash": 3, "kick": 10}, 512),
+    ("drrt18", 300, 12, {"k": 4, "buckets": 310, "stash": 1, "kick": 5}, 512),
+]
+
+
+@pytest.mark.parametrize("mode,pnum,ell,kw,count", PROBE_SHAPES)
+def test_failures_probes_tables(ctx, mode, pnum, ell, kw, count):
+    p, op = both(mode, pnum, ell, **kw)
+    seed, batch, first = 77, 3, 1000
+    f, probes, tables, st = ctx.launch_debug(p, ck.batch_key(seed, batch), first, count, tables=True)
+    bk = ock.batch_key(seed, batch)
+    of, oprobes, ost = ock.run_range(bk, first, count, op, probes=True)
+    assert list(f) == list(of)
+    assert (probes == oprobes).all()
+    assert int(probes.sum()) == ost.probes
+    for i in range(0, count, max(1, count // 16)):
+        failed, bins, _ = ock.trial_run(bk, first + i, op)
+        if not failed:
+            assert (tables[i] == bins).all(), i
+
+
+def test_headline_shape_prefix(ctx):
+    # n = 2^14, m = 24576, k = 3, kick 500, ell_i 16, seed 1, batch 0.
+    p, op = both("bgh25", 16384, 16, kick=500)
+    count = 1024
+    f, probes, _, st = ctx.launch_debug(p, ck.batch_key(1, 0), 0, count)
+    of, oprobes, _ = ock.run_range(ock.batch_key(1, 0), 0, count, op, probes=True)
+    assert list(f) == list(of)
+    assert (probes == oprobes).all()
+
+
+def test_chunking_does_not_change_results(ctx):
+    p, _ = both("drrt18", 16, 8)
+    rk = ck.batch_key(123, 3)
+    ref, _ = ctx.launch(p, rk, 0, 4096)
+    ctx.set_chunk(37)
+    try:
+        got, st = ctx.launch(p, rk, 0, 4096)
+        assert st.chunk == 37
+    finally:
+        ctx.set_chunk(0)
+    assert (got == ref).all()
