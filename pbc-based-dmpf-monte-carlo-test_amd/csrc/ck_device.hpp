@@ -11,7 +11,6 @@ namespace ck {
 
 constexpr uint32_t EMPTY = 0xffffffffu;
 constexpr uint32_t ORD_MASK = 0x3fffffffu;  // table entry = ordinal | choice << 30
-constexpr int TE_REPL = 32;                 // copies of T0 in LDS: lane L reads copy L % 32
 
 // Kernel-argument form of ck_params plus the host-precomputed divider.
 struct Shape {
@@ -25,12 +24,14 @@ struct RoundKeys {
 
 __device__ __forceinline__ uint32_t rotr(uint32_t x, uint32_t s) { return __builtin_amdgcn_alignbit(x, x, s); }
 
-// T0 replicated TE_REPL times: entry x of copy c sits at x*TE_REPL + c, so a
-// wave whose lane L reads copy L%32 touches bank L%32 whatever the byte index.
+// T0 in LDS, replicated 2^shift times: entry x of copy c sits at
+// (x << shift) | c.  With shift 5 a lane L reading copy L%32 touches bank L%32
+// whatever the byte index (no bank conflicts); shift 0 is the plain 1 KB table.
 struct LdsTab {
     const uint32_t* tab;
-    uint32_t lane;  // lane % TE_REPL
-    __device__ __forceinline__ uint32_t t(uint32_t x) const { return tab[(x << 5) | lane]; }
+    uint32_t lane;   // lane % 2^shift
+    uint32_t shift;
+    __device__ __forceinline__ uint32_t t(uint32_t x) const { return tab[(x << shift) | lane]; }
     __device__ __forceinline__ uint32_t s(uint32_t x) const { return (t(x) >> 16) & 255u; }
 };
 

@@ -1,15 +1,18 @@
 // ck_kernels.hip -- gfx950 kernels of the cuckoo Monte Carlo path and the
 // C-ABI entry points that launch them.
 //
-// Work per trial is split in two stages:
-//   build : one wave per trial.  Derives the PRP key and walk, builds the
-//           8*2^half round cache in LDS, evaluates bucket(o, w) for every
-//           ordinal and candidate (the bucket map), clears the trial's table.
-//           Full-lane integer work: AES T-table rounds and cached Feistel.
-//   chain : one lane per trial (HBM tier).  Runs the serial insertion over the
-//           HBM-resident map and table; one probe = one random table read (+
-//           one map read when an evicted item moves).  Latency-bound, so the
-//           launch keeps tens of thousands of chains in flight.
+// Each trial is (1) a full-lane build -- PRP key and walk derivation, the
+// 8*2^half round cache, the bucket map bucket(o, w) for every ordinal and
+// candidate, an empty table -- and (2) the serial insertion chain over that map
+// and table.  Two placements ("tiers"):
+//   HBM tier : k_build (one wave per trial) writes map and table to HBM, then
+//              k_chain runs one lane per trial; one probe is one random slot
+//              read (+ a map read when an evicted item moves).  The chains are
+//              latency-bound, so a launch keeps ~10^5 of them in flight.
+//   LDS tier : k_lds, one wave per trial with map, table and round cache in
+//              the wave's LDS region; the wave builds with all 64 lanes, then
+//              lane 0 runs the chain on chip.  Waves claim trials from an atomic
+//              cursor, so build-phase and chain-phase waves overlap on a CU.
 // Failures are appended with one ballot and one atomic per wave.
 #include <hip/hip_runtime.h>
 
@@ -18,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "ck_chain.hpp"
 #include "ck_device.hpp"
 #include "ck_internal.hpp"
 
@@ -25,33 +29,52 @@ namespace ck {
 
 __constant__ uint32_t c_te0[256];
 
+// Counter block at the start of the per-launch scratch (u32 words):
+//   [0] failures  [2..3] probe total (u64)  [4] LDS-tier work cursor
+constexpr uint32_t CNT_FAIL = 0, CNT_PROBES = 2, CNT_CURSOR = 4, CNT_WORDS = 16;
+
 // ------------------------------------------------------------------ kernels
 
-// Stages the replicated T0 into LDS; returns the lane's view.
-__device__ __forceinline__ LdsTab stage_tab(uint32_t* lds) {
-    for (uint32_t i = threadIdx.x; i < 256u * TE_REPL; i += blockDim.x) lds[i] = c_te0[i / TE_REPL];
+// Stages T0 (2^shift copies) into LDS; returns the lane's view.
+__device__ __forceinline__ LdsTab stage_tab(uint32_t* lds, uint32_t shift) {
+    const uint32_t n = 256u << shift;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = c_te0[i >> shift];
     __syncthreads();
-    return LdsTab{lds, threadIdx.x & (TE_REPL - 1)};
+    return LdsTab{lds, threadIdx.x & ((1u << shift) - 1u), shift};
 }
 
-// build: one wave per trial slot.  CACHED selects the round-cache form
-// (half <= 12); otherwise every bucket evaluation runs AES directly.
+__device__ __forceinline__ void derive_uniform(const LdsTab& T, const RoundKeys& bk, const Shape& S, uint32_t offset,
+                                               TrialSeed& ts) {
+    derive_trial(T, bk, S, offset, ts);
+#pragma unroll
+    for (int i = 0; i < 44; ++i) ts.rk[i] = __builtin_amdgcn_readfirstlane(ts.rk[i]);
+}
+
+// The wave's round cache: cache[r<<half | right] = F_r(right) & mask.
+__device__ __forceinline__ void build_cache(const LdsTab& T, const TrialSeed& ts, const Shape& S, uint16_t* cache,
+                                            uint32_t lane) {
+    const uint32_t entries = 8u << S.half, rmask = (1u << S.half) - 1u;
+    for (uint32_t e = lane; e < entries; e += 64)
+        cache[e] = static_cast<uint16_t>(aes_last(T, ts.rk, 0x43455350u, e >> S.half, S.half, e & rmask) & S.mask);
+    wave_lds_sync();
+}
+
+// build (HBM tier): one wave per trial slot.  CACHED selects the round-cache
+// form (half <= 12); otherwise every bucket evaluation runs AES directly.
 template <int K, bool CACHED>
 __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
-                                               uint32_t* __restrict__ map, size_t map_stride,
+                                               uint32_t tab_shift, uint32_t* __restrict__ map, size_t map_stride,
                                                uint32_t* __restrict__ bins, size_t bins_stride,
                                                uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
                                                uint32_t* __restrict__ key_out) {
     extern __shared__ uint32_t lds[];
-    const LdsTab T = stage_tab(lds);
+    const LdsTab T = stage_tab(lds, tab_shift);
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t slot = blockIdx.x * waves + wave;
     if (slot >= count) return;  // wave-uniform; no block barrier follows
 
     TrialSeed ts;
-    derive_trial(T, bk, S, first + slot, ts);
-#pragma unroll
-    for (int i = 0; i < 44; ++i) ts.rk[i] = __builtin_amdgcn_readfirstlane(ts.rk[i]);
+    derive_uniform(T, bk, S, first + slot, ts);
     if (key_out && lane < 44) {
         uint32_t v = 0;
 #pragma unroll
@@ -61,11 +84,8 @@ __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t f
 
     uint32_t* m = map + slot * map_stride;
     if constexpr (CACHED) {
-        uint16_t* cache = reinterpret_cast<uint16_t*>(lds + 256 * TE_REPL) + (size_t(wave) << (S.half + 3));
-        const uint32_t entries = 8u << S.half, rmask = (1u << S.half) - 1u;
-        for (uint32_t e = lane; e < entries; e += 64)
-            cache[e] = static_cast<uint16_t>(aes_last(T, ts.rk, 0x43455350u, e >> S.half, S.half, e & rmask) & S.mask);
-        wave_lds_sync();
+        uint16_t* cache = reinterpret_cast<uint16_t*>(lds + (256u << tab_shift)) + (size_t(wave) << (S.half + 3));
+        build_cache(T, ts, S, cache, lane);
         for (uint32_t o = lane; o < S.pnum; o += 64) {
             const uint64_t el = element_of(S, ts.shift, ts.step, o);
 #pragma unroll
@@ -87,131 +107,14 @@ __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t f
     }
 }
 
-// BGH25 random-walk insertion (cuckoo_core.hpp:123-133) with the k/stash
-// extension: fail (or stash the item in hand) once collisions exceed kick.
-template <int K>
-__device__ __forceinline__ bool chain_bgh(const Shape& S, const uint32_t* __restrict__ m, uint32_t* __restrict__ t,
-                                          uint64_t walk, uint64_t rnd, uint32_t& probes) {
-    uint32_t item = 0, cur = 0, coll = 0, stash_left = S.stash, pr = 0;
-    bool failed = false;
-    while (item < S.pnum) {
-        if (rnd < K) rnd = splitmix(walk);
-        const uint32_t which = take_choice<K>(rnd);
-        const uint32_t b = m[cur * K + which];
-        const uint32_t old = t[b];
-        ++pr;
-        if (old == EMPTY) {
-            t[b] = cur | (which << 30);
-            cur = ++item;
-            coll = 0;
-            continue;
-        }
-        if (++coll > S.kick) {
-            if (stash_left) {
-                --stash_left;
-                cur = ++item;
-                coll = 0;
-                continue;
-            }
-            failed = true;
-            break;
-        }
-        t[b] = cur | (which << 30);
-        cur = old & ORD_MASK;
-    }
-    probes = pr;
-    return failed;
-}
-
-template <int K>
-__device__ __forceinline__ uint32_t pick(const uint32_t (&c)[K], uint32_t i) {
-    uint32_t v = c[0];
-#pragma unroll
-    for (int w = 1; w < K; ++w) v = (i == uint32_t(w)) ? c[w] : v;
-    return v;
-}
-
-// DRRT18 deterministic insertion (cuckoo_core.hpp:134-158) with the k/stash
-// extension: try candidates 0..k-1, then evict from candidate 0; the displaced
-// item cycles its stored choice to the first bucket != the one it left.
-template <int K>
-__device__ __forceinline__ bool chain_drrt(const Shape& S, const uint32_t* __restrict__ m, uint32_t* __restrict__ t,
-                                           uint32_t& probes) {
-    uint32_t pr = 0, stash_left = S.stash;
-    bool failed = false;
-    for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
-        uint32_t c[K];
-#pragma unroll
-        for (int w = 0; w < K; ++w) c[w] = m[item * K + w];
-        bool placed = false;
-#pragma unroll
-        for (int w = 0; w < K; ++w) {
-            if (placed) break;
-            ++pr;
-            if (t[c[w]] == EMPTY) {
-                t[c[w]] = item | (uint32_t(w) << 30);
-                placed = true;
-            }
-        }
-        if (placed) continue;
-        uint32_t cur = item, choice = 0, target = c[0], kicks = 0;
-        for (;;) {
-            const uint32_t disp = t[target];
-            ++pr;
-            if (disp == EMPTY) {
-                t[target] = cur | (choice << 30);
-                break;
-            }
-            if (kicks >= S.kick) {
-                if (stash_left) {
-                    --stash_left;
-                    break;
-                }
-                failed = true;
-                break;
-            }
-            t[target] = cur | (choice << 30);
-            cur = disp & ORD_MASK;
-            choice = disp >> 30;
-            const uint32_t prev = target;
-            uint32_t cc[K];
-#pragma unroll
-            for (int w = 0; w < K; ++w) cc[w] = m[cur * K + w];
-            bool found = false;
-#pragma unroll
-            for (int s = 0; s < K; ++s) {
-                if (found) break;
-                choice = (choice + 1 == K) ? 0 : choice + 1;
-                const uint32_t nb = pick<K>(cc, choice);
-                if (nb != prev) {
-                    target = nb;
-                    found = true;
-                }
-            }
-            if (!found) {
-                if (stash_left) {
-                    --stash_left;
-                    break;
-                }
-                failed = true;
-                break;
-            }
-            ++kicks;
-        }
-    }
-    probes = pr;
-    return failed;
-}
-
 // chain (HBM tier): one lane per trial slot.
 template <int K, bool BGH>
 __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t count,
                                                const uint32_t* __restrict__ map, size_t map_stride,
                                                uint32_t* __restrict__ bins, size_t bins_stride,
                                                const uint64_t* __restrict__ walks, const uint64_t* __restrict__ rnds,
-                                               uint32_t* __restrict__ fail_count, uint32_t* __restrict__ fail_offsets,
-                                               uint32_t* __restrict__ probes_out,
-                                               unsigned long long* __restrict__ probe_total) {
+                                               uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
+                                               uint32_t* __restrict__ probes_out) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     bool failed = false;
     uint32_t pr = 0;
@@ -224,8 +127,70 @@ __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t
             failed = chain_drrt<K>(S, m, t, pr);
         if (probes_out) probes_out[slot] = pr;
     }
-    append_failures(failed, first + slot, fail_count, fail_offsets);
-    add_wave_total(pr, probe_total);
+    append_failures(failed, first + slot, counters + CNT_FAIL, fail_offsets);
+    add_wave_total(pr, reinterpret_cast<unsigned long long*>(counters + CNT_PROBES));
+}
+
+// LDS tier: a persistent wave per trial at a time.  Wave region layout:
+// [round cache 8<<half u16][map pnum*K M][table buckets E].
+template <int K, bool BGH, class M, class E>
+__global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t first, uint32_t count, uint32_t tab_shift,
+                                             uint32_t wave_bytes, uint32_t cache_bytes, uint32_t map_bytes,
+                                             uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
+                                             uint32_t* __restrict__ probes_out, uint32_t* __restrict__ tables_out) {
+    extern __shared__ uint32_t lds[];
+    const LdsTab T = stage_tab(lds, tab_shift);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    char* region = reinterpret_cast<char*>(lds + (256u << tab_shift)) + size_t(wave) * wave_bytes;
+    uint16_t* cache = reinterpret_cast<uint16_t*>(region);
+    M* map = reinterpret_cast<M*>(region + cache_bytes);
+    E* table = reinterpret_cast<E*>(region + cache_bytes + map_bytes);
+    // Every iteration starts and ends wave-converged: the only lane-divergent
+    // region is the chain itself.  Single-lane side effects (claiming a slot,
+    // publishing the outcome) are written as whole-wave operations on
+    // wave-uniform values, so the compiler has no lane-0 branch to thread from
+    // the end of one iteration into the start of the next.
+    // The cursor is monotone, so no wave can claim more than count + 1 times.
+    for (uint32_t iter = 0; iter <= count; ++iter) {
+        const uint32_t claimed = atomicAdd(counters + CNT_CURSOR, lane == 0 ? 1u : 0u);
+        const uint32_t slot = __builtin_amdgcn_readfirstlane(claimed);
+        if (slot >= count) break;
+        TrialSeed ts;
+        derive_uniform(T, bk, S, first + slot, ts);
+        build_cache(T, ts, S, cache, lane);
+        for (uint32_t o = lane; o < S.pnum; o += 64) {
+            const uint64_t el = element_of(S, ts.shift, ts.step, o);
+#pragma unroll
+            for (int w = 0; w < K; ++w) map[o * K + w] = static_cast<M>(feistel_cached(cache, S, el, K, w));
+        }
+        for (uint32_t i = lane; i < S.buckets; i += 64) table[i] = empty_entry<E>();
+        wave_lds_sync();
+        bool failed_lane = false;
+        uint32_t pr_lane = 0;
+        if (lane == 0) {
+            if constexpr (BGH)
+                failed_lane = chain_bgh<K>(S, map, table, ts.walk, ts.rnd, pr_lane);
+            else
+                failed_lane = chain_drrt<K>(S, map, table, pr_lane);
+        }
+        wave_lds_sync();
+        const uint32_t failed = __builtin_amdgcn_readfirstlane(failed_lane ? 1u : 0u);
+        const uint32_t pr = __builtin_amdgcn_readfirstlane(pr_lane);
+        if (tables_out) {
+            uint32_t* out = tables_out + size_t(slot) * S.buckets;
+            for (uint32_t i = lane; i < S.buckets; i += 64) {
+                const E e = table[i];
+                out[i] = e == empty_entry<E>() ? EMPTY : (entry_ordinal<E>(e) | (entry_choice<E>(e) << 30));
+            }
+        }
+        if (probes_out) probes_out[slot] = pr;  // same value from every lane
+        if (failed) {
+            const uint32_t at = __builtin_amdgcn_readfirstlane(atomicAdd(counters + CNT_FAIL, lane == 0 ? 1u : 0u));
+            fail_offsets[at] = first + slot;  // same value from every lane
+        }
+        atomicAdd(reinterpret_cast<unsigned long long*>(counters + CNT_PROBES),
+                  static_cast<unsigned long long>(lane == 0 ? pr : 0u));
+    }
 }
 
 __global__ void k_zero(uint32_t* p, uint32_t n) {
@@ -234,10 +199,11 @@ __global__ void k_zero(uint32_t* p, uint32_t n) {
 }
 
 // FIPS-197 C.1 through the device key schedule and T-table AES on every lane
-// of one wave (aes_kat / cached_aes_kat, cuckoo_bench.cu:322-354).
-__global__ __launch_bounds__(64) void k_aes_kat(uint32_t* out) {
+// of one wave, with the replicated (shift 5) and plain (shift 0) tables
+// (aes_kat / cached_aes_kat, cuckoo_bench.cu:322-354).
+__global__ __launch_bounds__(64) void k_aes_kat(uint32_t* out, uint32_t tab_shift) {
     extern __shared__ uint32_t lds[];
-    const LdsTab T = stage_tab(lds);
+    const LdsTab T = stage_tab(lds, tab_shift);
     const uint32_t key[4] = {0x00010203u, 0x04050607u, 0x08090a0bu, 0x0c0d0e0fu};
     uint32_t rk[44], o[4];
     expand_key_dev(T, key, rk);
@@ -247,10 +213,10 @@ __global__ __launch_bounds__(64) void k_aes_kat(uint32_t* out) {
 
 // ---------------------------------------------------------------- host side
 
-#define CKH(expr)                                                                              \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) return fail(CK_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+#define CKH(expr)                                                                                              \
+    do {                                                                                                       \
+        hipError_t e_ = (expr);                                                                                \
+        if (e_ != hipSuccess) return fail(CK_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
     } while (0)
 
 struct Workspace {
@@ -266,7 +232,9 @@ struct Ctx {
     uint32_t tier = CK_TIER_AUTO;
     uint32_t chunk_cap = 0;
     size_t budget = size_t(48) << 30;  // HBM budget for per-slot maps and tables
-    size_t lds_max = 64 * 1024;        // dynamic LDS per block the device allows
+    size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
+    size_t lds_cu = 64 * 1024;         // LDS per CU
+    uint32_t cus = 1;
 };
 
 static Shape make_shape(const ck_params& p) {
@@ -296,21 +264,23 @@ static int check_params(const ck_params* p) {
     return CK_OK;
 }
 
-// Largest round cache a build wave keeps in LDS next to the 32 KB T-table.
+// Round caches above 2^12 entries per round do not pay for themselves in LDS.
 static constexpr uint32_t kMaxCacheHalf = 12;
+static constexpr uint32_t kRepl = 5;  // T0 copies = 2^5 for conflict-free lookups
+static size_t tab_bytes(uint32_t shift) { return size_t(256) << shift << 2; }
 
 struct BuildPlan {
     bool cached;
     uint32_t waves;  // trials per build block
     size_t lds;
 };
-static BuildPlan plan_build(const ck_params& p, size_t lds_max) {
+static BuildPlan plan_build(const ck_params& p, size_t lds_block) {
     BuildPlan b{};
-    const size_t tab = 256 * TE_REPL * sizeof(uint32_t);
+    const size_t tab = tab_bytes(kRepl);
     const size_t cache = (size_t(8) << p.half_bits) * sizeof(uint16_t);
-    b.cached = p.half_bits <= kMaxCacheHalf && tab + cache <= lds_max;
+    b.cached = p.half_bits <= kMaxCacheHalf && tab + cache <= lds_block;
     if (b.cached) {
-        b.waves = static_cast<uint32_t>(std::min<size_t>(4, (lds_max - tab) / cache));
+        b.waves = static_cast<uint32_t>(std::min<size_t>(4, (lds_block - tab) / cache));
         b.lds = tab + b.waves * cache;
     } else {
         b.waves = 4;
@@ -319,44 +289,94 @@ static BuildPlan plan_build(const ck_params& p, size_t lds_max) {
     return b;
 }
 
+struct LdsPlan {
+    bool ok = false, map16 = false, bins16 = false;
+    uint32_t waves = 0, tab_shift = 0, blocks_per_cu = 0;
+    size_t cache_b = 0, map_b = 0, bins_b = 0, wave_b = 0, lds = 0;
+};
+static LdsPlan plan_lds(const ck_params& p, size_t lds_block, size_t lds_cu) {
+    LdsPlan L;
+    if (p.half_bits > kMaxCacheHalf) return L;
+    auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+    L.map16 = p.buckets <= 65536;
+    L.bins16 = p.pnum < 16384 || (p.pnum == 16384 && p.k <= 3);
+    L.cache_b = a16((size_t(8) << p.half_bits) * 2);
+    L.map_b = a16(size_t(p.pnum) * p.k * (L.map16 ? 2 : 4));
+    L.bins_b = a16(size_t(p.buckets) * (L.bins16 ? 2 : 4));
+    L.wave_b = L.cache_b + L.map_b + L.bins_b;
+    uint32_t best = 0;
+    for (uint32_t shift : {kRepl, 0u}) {
+        for (uint32_t w = 1; w <= 8; ++w) {
+            const size_t blk = tab_bytes(shift) + w * L.wave_b;
+            if (blk > lds_block) break;
+            const uint32_t per_cu = static_cast<uint32_t>(lds_cu / blk);
+            const uint32_t waves_cu = std::min<uint32_t>(per_cu * w, 32);
+            if (waves_cu > best) {
+                best = waves_cu;
+                L.waves = w;
+                L.tab_shift = shift;
+                L.blocks_per_cu = per_cu;
+                L.lds = blk;
+            }
+        }
+    }
+    L.ok = best > 0;
+    return L;
+}
+
+// Shapes whose chains stay on chip with at least this many resident trials
+// per CU use the LDS tier by default; larger tables go to HBM.
+static constexpr uint32_t kAutoLdsMinWaves = 4;
+
+static uint32_t choose_tier(const Ctx& c, const ck_params& p) {
+    if (c.tier != CK_TIER_AUTO) return c.tier;
+    const LdsPlan L = plan_lds(p, c.lds_block, c.lds_cu);
+    return L.ok && L.blocks_per_cu * L.waves >= kAutoLdsMinWaves ? CK_TIER_LDS : CK_TIER_HBM;
+}
+
 struct Layout {
     size_t map_stride, bins_stride;  // u32 entries per slot
-    size_t off_map, off_bins, off_walk, off_rnd, off_probes, off_fail, off_count, off_key, total;
+    size_t off_cnt, off_map, off_bins, off_walk, off_rnd, off_probes, off_fail, off_key, off_tables, total;
 };
-static Layout layout_for(const ck_params& p, uint32_t chunk) {
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+static Layout layout_for(const ck_params& p, uint32_t chunk, bool hbm, bool tables) {
     Layout L{};
-    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    L.map_stride = (size_t(p.pnum) * p.k + 63) & ~size_t(63);
-    L.bins_stride = (size_t(p.buckets) + 63) & ~size_t(63);
+    L.map_stride = hbm ? (size_t(p.pnum) * p.k + 63) & ~size_t(63) : 0;
+    L.bins_stride = hbm ? (size_t(p.buckets) + 63) & ~size_t(63) : 0;
     size_t o = 0;
+    L.off_cnt = o;
+    o = align256(o + CNT_WORDS * 4);
     L.off_map = o;
-    o = align(o + L.map_stride * chunk * 4);
+    o = align256(o + L.map_stride * chunk * 4);
     L.off_bins = o;
-    o = align(o + L.bins_stride * chunk * 4);
+    o = align256(o + L.bins_stride * chunk * 4);
     L.off_walk = o;
-    o = align(o + size_t(chunk) * 8);
+    o = align256(o + (hbm ? size_t(chunk) * 8 : 0));
     L.off_rnd = o;
-    o = align(o + size_t(chunk) * 8);
+    o = align256(o + (hbm ? size_t(chunk) * 8 : 0));
     L.off_probes = o;
-    o = align(o + size_t(chunk) * 4);
+    o = align256(o + size_t(chunk) * 4);
     L.off_fail = o;
-    o = align(o + size_t(chunk) * 4);
-    L.off_count = o;
-    o = align(o + 64);
+    o = align256(o + size_t(chunk) * 4);
     L.off_key = o;
-    o = align(o + size_t(chunk) * 44 * 4);
+    o = align256(o + (hbm ? size_t(chunk) * 44 * 4 : 0));
+    L.off_tables = o;
+    o = align256(o + (tables && !hbm ? size_t(chunk) * p.buckets * 4 : 0));
     L.total = o;
     return L;
 }
 
-static uint32_t default_chunk(const Ctx& c, const ck_params& p, uint32_t count) {
-    const size_t per_slot = ((size_t(p.pnum) * p.k + 63) & ~size_t(63)) * 4 + ((size_t(p.buckets) + 63) & ~size_t(63)) * 4 +
-                            16 + 8 + 176;
-    size_t fit = c.budget / per_slot;
-    uint32_t target = 1u << 17;  // lanes in flight: 2048 waves, 8 per CU
-    if (c.chunk_cap) target = c.chunk_cap;
-    uint32_t chunk = static_cast<uint32_t>(std::min<size_t>({fit, size_t(target), size_t(count)}));
-    return std::max<uint32_t>(chunk, 1);
+static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count) {
+    const size_t per_slot = ((size_t(p.pnum) * p.k + 63) & ~size_t(63)) * 4 +
+                            ((size_t(p.buckets) + 63) & ~size_t(63)) * 4 + 16 + 8 + 176;
+    const size_t fit = std::max<size_t>(1, c.budget / per_slot);
+    const uint32_t target = c.chunk_cap ? c.chunk_cap : (1u << 17);  // ~2048 waves in flight
+    return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>({fit, size_t(target), size_t(count)})));
+}
+static uint32_t lds_chunk(const Ctx& c, uint32_t count, bool tables, const ck_params& p) {
+    uint32_t target = c.chunk_cap ? c.chunk_cap : (1u << 20);
+    if (tables) target = std::min<uint32_t>(target, static_cast<uint32_t>(std::max<size_t>(1, (size_t(1) << 30) / (size_t(p.buckets) * 4))));
+    return std::max<uint32_t>(1, std::min(target, count));
 }
 
 static int ensure_ws(Ctx& c, size_t bytes) {
@@ -375,65 +395,128 @@ static void launch_build(const BuildPlan& bp, const Shape& S, const RoundKeys& b
                          uint64_t* rnds, uint32_t* keys, hipStream_t st) {
     const uint32_t blocks = (count + bp.waves - 1) / bp.waves;
     if (bp.cached)
-        hipLaunchKernelGGL((k_build<K, true>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, S, bk, first, count, map,
-                           map_stride, bins, bins_stride, walks, rnds, keys);
+        hipLaunchKernelGGL((k_build<K, true>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, S, bk, first, count,
+                           kRepl, map, map_stride, bins, bins_stride, walks, rnds, keys);
     else
-        hipLaunchKernelGGL((k_build<K, false>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, S, bk, first, count, map,
-                           map_stride, bins, bins_stride, walks, rnds, keys);
+        hipLaunchKernelGGL((k_build<K, false>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, S, bk, first, count,
+                           kRepl, map, map_stride, bins, bins_stride, walks, rnds, keys);
 }
 
 template <int K>
 static void launch_chain(bool bgh, const Shape& S, uint32_t first, uint32_t count, const uint32_t* map,
                          size_t map_stride, uint32_t* bins, size_t bins_stride, const uint64_t* walks,
-                         const uint64_t* rnds, uint32_t* fc, uint32_t* fo, uint32_t* probes,
-                         unsigned long long* ptotal, hipStream_t st) {
+                         const uint64_t* rnds, uint32_t* cnt, uint32_t* fo, uint32_t* probes, hipStream_t st) {
     const uint32_t blocks = (count + 255) / 256;
     if (bgh)
         hipLaunchKernelGGL((k_chain<K, true>), dim3(blocks), dim3(256), 0, st, S, first, count, map, map_stride, bins,
-                           bins_stride, walks, rnds, fc, fo, probes, ptotal);
+                           bins_stride, walks, rnds, cnt, fo, probes);
     else
         hipLaunchKernelGGL((k_chain<K, false>), dim3(blocks), dim3(256), 0, st, S, first, count, map, map_stride, bins,
-                           bins_stride, walks, rnds, fc, fo, probes, ptotal);
+                           bins_stride, walks, rnds, cnt, fo, probes);
 }
 
-static int set_lds_limits(size_t lds_max) {
-    // Opt every build variant into the device's full LDS (needed above 64 KB).
-    const int lim = static_cast<int>(lds_max);
-#define OPT(K)                                                                                             \
-    CKH(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_build<K, true>),                              \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lim));                            \
-    CKH(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_build<K, false>),                             \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    OPT(2) OPT(3) OPT(4)
-#undef OPT
+template <int K, bool BGH>
+static void launch_lds_kb(const LdsPlan& L, uint32_t blocks, const Shape& S, const RoundKeys& bk, uint32_t first,
+                          uint32_t count, uint32_t* cnt, uint32_t* fo, uint32_t* probes, uint32_t* tables,
+                          hipStream_t st) {
+    const dim3 grid(blocks), block(64 * L.waves);
+    const uint32_t wb = static_cast<uint32_t>(L.wave_b), cb = static_cast<uint32_t>(L.cache_b),
+                   mb = static_cast<uint32_t>(L.map_b);
+#define LDS_ARGS S, bk, first, count, L.tab_shift, wb, cb, mb, cnt, fo, probes, tables
+    if (L.map16 && L.bins16)
+        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint16_t>), grid, block, L.lds, st, LDS_ARGS);
+    else if (L.map16)
+        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint32_t>), grid, block, L.lds, st, LDS_ARGS);
+    else if (L.bins16)
+        hipLaunchKernelGGL((k_lds<K, BGH, uint32_t, uint16_t>), grid, block, L.lds, st, LDS_ARGS);
+    else
+        hipLaunchKernelGGL((k_lds<K, BGH, uint32_t, uint32_t>), grid, block, L.lds, st, LDS_ARGS);
+#undef LDS_ARGS
+}
+template <int K>
+static void launch_lds(bool bgh, const LdsPlan& L, uint32_t blocks, const Shape& S, const RoundKeys& bk,
+                       uint32_t first, uint32_t count, uint32_t* cnt, uint32_t* fo, uint32_t* probes,
+                       uint32_t* tables, hipStream_t st) {
+    if (bgh)
+        launch_lds_kb<K, true>(L, blocks, S, bk, first, count, cnt, fo, probes, tables, st);
+    else
+        launch_lds_kb<K, false>(L, blocks, S, bk, first, count, cnt, fo, probes, tables, st);
+}
+
+template <class Fn>
+static int for_each_lds_kernel(Fn&& fn) {
+#define ONE(K, B, M, E)                                                              \
+    {                                                                                \
+        const int rc = fn(reinterpret_cast<const void*>(&k_lds<K, B, M, E>));        \
+        if (rc) return rc;                                                           \
+    }
+#define ALL_ME(K, B) ONE(K, B, uint16_t, uint16_t) ONE(K, B, uint16_t, uint32_t) ONE(K, B, uint32_t, uint16_t) ONE(K, B, uint32_t, uint32_t)
+    ALL_ME(2, true) ALL_ME(2, false) ALL_ME(3, true) ALL_ME(3, false) ALL_ME(4, true) ALL_ME(4, false)
+#undef ALL_ME
+#undef ONE
     return CK_OK;
 }
 
-// One pass over trials [first, first+count) of the batch: build + chain per
-// chunk; failing offsets are appended to out (unsorted).
-static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count,
-                   std::vector<uint32_t>& out, uint32_t* probes_host, uint32_t* tables_host, uint32_t* keys_host,
-                   uint64_t* walks_host, uint64_t* rnds_host, uint32_t* map_host, ck_stats* st) {
+static int set_lds_limits(size_t lds_block) {
+    const int lim = static_cast<int>(lds_block);
+    auto opt = [&](const void* f) -> int {
+        CKH(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+        return CK_OK;
+    };
+    int rc;
+    for (const void* f : {reinterpret_cast<const void*>(&k_build<2, true>), reinterpret_cast<const void*>(&k_build<2, false>),
+                          reinterpret_cast<const void*>(&k_build<3, true>), reinterpret_cast<const void*>(&k_build<3, false>),
+                          reinterpret_cast<const void*>(&k_build<4, true>), reinterpret_cast<const void*>(&k_build<4, false>)})
+        if ((rc = opt(f))) return rc;
+    return for_each_lds_kernel(opt);
+}
+
+struct HostOut {
+    std::vector<uint32_t>* fails;
+    uint32_t* probes;   // per trial, may be null
+    uint32_t* tables;   // per trial x buckets, may be null
+    uint32_t* keys;     // HBM tier debug: per trial 44 words
+    uint64_t* walks;
+    uint64_t* rnds;
+    uint32_t* map;      // HBM tier debug: per trial pnum*k
+};
+
+static int collect(Ctx& c, const uint32_t* cnt, const uint32_t* fo, uint32_t n, HostOut& out, ck_stats* st) {
+    uint32_t counters[4] = {0, 0, 0, 0};
+    CKH(hipMemcpyAsync(counters, cnt, sizeof counters, hipMemcpyDeviceToHost, c.stream));
+    CKH(hipStreamSynchronize(c.stream));
+    const uint32_t nf = counters[CNT_FAIL];
+    if (nf > n) return fail(CK_ERANGE, "device failure count out of range");
+    if (nf) {
+        const size_t at = out.fails->size();
+        out.fails->resize(at + nf);
+        CKH(hipMemcpy(out.fails->data() + at, fo, size_t(nf) * 4, hipMemcpyDeviceToHost));
+    }
+    if (st) st->probes += (uint64_t(counters[CNT_PROBES + 1]) << 32) | counters[CNT_PROBES];
+    return CK_OK;
+}
+
+// HBM tier: build + chain per chunk of trial slots.
+static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
+                   ck_stats* st) {
     const Shape S = make_shape(p);
     RoundKeys bk;
     std::memcpy(bk.w, rk, sizeof bk.w);
-    const uint32_t chunk = default_chunk(c, p, count);
-    const Layout L = layout_for(p, chunk);
+    const uint32_t chunk = hbm_chunk(c, p, count);
+    const Layout L = layout_for(p, chunk, true, false);
     int rc = ensure_ws(c, L.total);
     if (rc) return rc;
     char* base = static_cast<char*>(c.ws.base);
+    auto* cnt = reinterpret_cast<uint32_t*>(base + L.off_cnt);
     auto* map = reinterpret_cast<uint32_t*>(base + L.off_map);
     auto* bins = reinterpret_cast<uint32_t*>(base + L.off_bins);
     auto* walks = reinterpret_cast<uint64_t*>(base + L.off_walk);
     auto* rnds = reinterpret_cast<uint64_t*>(base + L.off_rnd);
     auto* probes = reinterpret_cast<uint32_t*>(base + L.off_probes);
     auto* fo = reinterpret_cast<uint32_t*>(base + L.off_fail);
-    auto* fc = reinterpret_cast<uint32_t*>(base + L.off_count);
     auto* keys = reinterpret_cast<uint32_t*>(base + L.off_key);
-    const BuildPlan bp = plan_build(p, c.lds_max);
+    const BuildPlan bp = plan_build(p, c.lds_block);
     const bool bgh = p.mode == CK_MODE_BGH25;
-    const bool want_probes = probes_host != nullptr;
-    std::vector<uint32_t> chunk_fail(chunk);
     if (st) {
         st->tier = CK_TIER_HBM;
         st->chunk = chunk;
@@ -441,43 +524,35 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     for (uint32_t done = 0; done < count;) {
         const uint32_t n = std::min(chunk, count - done);
         const uint32_t at = first + done;
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c.stream, fc, 16u);
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c.stream, cnt, CNT_WORDS);
         CKH(hipEventRecord(c.ev[0], c.stream));
+        uint32_t* kout = out.keys ? keys : nullptr;
         switch (p.k) {
-            case 2: launch_build<2>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, keys_host ? keys : nullptr, c.stream); break;
-            case 3: launch_build<3>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, keys_host ? keys : nullptr, c.stream); break;
-            default: launch_build<4>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, keys_host ? keys : nullptr, c.stream); break;
+            case 2: launch_build<2>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, kout, c.stream); break;
+            case 3: launch_build<3>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, kout, c.stream); break;
+            default: launch_build<4>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, kout, c.stream); break;
         }
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev[1], c.stream));
-        auto* ptot = reinterpret_cast<unsigned long long*>(fc + 2);
-        uint32_t* pr_out = want_probes ? probes : nullptr;
+        uint32_t* pout = out.probes ? probes : nullptr;
         switch (p.k) {
-            case 2: launch_chain<2>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, fc, fo, pr_out, ptot, c.stream); break;
-            case 3: launch_chain<3>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, fc, fo, pr_out, ptot, c.stream); break;
-            default: launch_chain<4>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, fc, fo, pr_out, ptot, c.stream); break;
+            case 2: launch_chain<2>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, cnt, fo, pout, c.stream); break;
+            case 3: launch_chain<3>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, cnt, fo, pout, c.stream); break;
+            default: launch_chain<4>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, cnt, fo, pout, c.stream); break;
         }
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev[2], c.stream));
-        uint32_t counters[4] = {0, 0, 0, 0};  // failures, pad, probe total (u64)
-        CKH(hipMemcpyAsync(counters, fc, sizeof counters, hipMemcpyDeviceToHost, c.stream));
-        CKH(hipStreamSynchronize(c.stream));
-        const uint32_t nf = counters[0];
-        if (st) st->probes += (uint64_t(counters[3]) << 32) | counters[2];
-        if (nf > n) return fail(CK_ERANGE, "device failure count out of range");
-        if (nf) {
-            CKH(hipMemcpy(chunk_fail.data(), fo, size_t(nf) * 4, hipMemcpyDeviceToHost));
-            out.insert(out.end(), chunk_fail.begin(), chunk_fail.begin() + nf);
-        }
-        if (probes_host) CKH(hipMemcpy(probes_host + done, probes, size_t(n) * 4, hipMemcpyDeviceToHost));
-        if (tables_host)
-            CKH(hipMemcpy2D(tables_host + size_t(done) * p.buckets, size_t(p.buckets) * 4, bins, L.bins_stride * 4,
+        rc = collect(c, cnt, fo, n, out, st);
+        if (rc) return rc;
+        if (out.probes) CKH(hipMemcpy(out.probes + done, probes, size_t(n) * 4, hipMemcpyDeviceToHost));
+        if (out.tables)
+            CKH(hipMemcpy2D(out.tables + size_t(done) * p.buckets, size_t(p.buckets) * 4, bins, L.bins_stride * 4,
                             size_t(p.buckets) * 4, n, hipMemcpyDeviceToHost));
-        if (keys_host) CKH(hipMemcpy(keys_host + size_t(done) * 44, keys, size_t(n) * 44 * 4, hipMemcpyDeviceToHost));
-        if (walks_host) CKH(hipMemcpy(walks_host + done, walks, size_t(n) * 8, hipMemcpyDeviceToHost));
-        if (rnds_host) CKH(hipMemcpy(rnds_host + done, rnds, size_t(n) * 8, hipMemcpyDeviceToHost));
-        if (map_host)
-            CKH(hipMemcpy2D(map_host + size_t(done) * p.pnum * p.k, size_t(p.pnum) * p.k * 4, map, L.map_stride * 4,
+        if (out.keys) CKH(hipMemcpy(out.keys + size_t(done) * 44, keys, size_t(n) * 44 * 4, hipMemcpyDeviceToHost));
+        if (out.walks) CKH(hipMemcpy(out.walks + done, walks, size_t(n) * 8, hipMemcpyDeviceToHost));
+        if (out.rnds) CKH(hipMemcpy(out.rnds + done, rnds, size_t(n) * 8, hipMemcpyDeviceToHost));
+        if (out.map)
+            CKH(hipMemcpy2D(out.map + size_t(done) * p.pnum * p.k, size_t(p.pnum) * p.k * 4, map, L.map_stride * 4,
                             size_t(p.pnum) * p.k * 4, n, hipMemcpyDeviceToHost));
         if (st) {
             float b = 0, ch = 0;
@@ -485,7 +560,61 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
             CKH(hipEventElapsedTime(&ch, c.ev[1], c.ev[2]));
             st->build_ms += b;
             st->chain_ms += ch;
-            st->launches += 3;
+            st->launches += 2;
+        }
+        done += n;
+    }
+    return CK_OK;
+}
+
+// LDS tier: one fused persistent launch per chunk.
+static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
+                   ck_stats* st) {
+    const LdsPlan plan = plan_lds(p, c.lds_block, c.lds_cu);
+    if (!plan.ok) return fail(CK_EINVAL, "shape does not fit the LDS tier (half_bits > 12 or table too large)");
+    const Shape S = make_shape(p);
+    RoundKeys bk;
+    std::memcpy(bk.w, rk, sizeof bk.w);
+    const uint32_t chunk = lds_chunk(c, count, out.tables != nullptr, p);
+    const Layout L = layout_for(p, chunk, false, out.tables != nullptr);
+    int rc = ensure_ws(c, L.total);
+    if (rc) return rc;
+    char* base = static_cast<char*>(c.ws.base);
+    auto* cnt = reinterpret_cast<uint32_t*>(base + L.off_cnt);
+    auto* probes = reinterpret_cast<uint32_t*>(base + L.off_probes);
+    auto* fo = reinterpret_cast<uint32_t*>(base + L.off_fail);
+    auto* tables = reinterpret_cast<uint32_t*>(base + L.off_tables);
+    const bool bgh = p.mode == CK_MODE_BGH25;
+    const uint32_t resident = plan.blocks_per_cu * c.cus;
+    if (st) {
+        st->tier = CK_TIER_LDS;
+        st->chunk = chunk;
+    }
+    for (uint32_t done = 0; done < count;) {
+        const uint32_t n = std::min(chunk, count - done);
+        const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(resident, (n + plan.waves - 1) / plan.waves));
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c.stream, cnt, CNT_WORDS);
+        CKH(hipEventRecord(c.ev[0], c.stream));
+        uint32_t* pout = out.probes ? probes : nullptr;
+        uint32_t* tout = out.tables ? tables : nullptr;
+        switch (p.k) {
+            case 2: launch_lds<2>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, c.stream); break;
+            case 3: launch_lds<3>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, c.stream); break;
+            default: launch_lds<4>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, c.stream); break;
+        }
+        CKH(hipGetLastError());
+        CKH(hipEventRecord(c.ev[1], c.stream));
+        rc = collect(c, cnt, fo, n, out, st);
+        if (rc) return rc;
+        if (out.probes) CKH(hipMemcpy(out.probes + done, probes, size_t(n) * 4, hipMemcpyDeviceToHost));
+        if (out.tables)
+            CKH(hipMemcpy(out.tables + size_t(done) * p.buckets, tables, size_t(n) * p.buckets * 4,
+                          hipMemcpyDeviceToHost));
+        if (st) {
+            float f = 0;
+            CKH(hipEventElapsedTime(&f, c.ev[0], c.ev[1]));
+            st->fused_ms += f;
+            st->launches += 1;
         }
         done += n;
     }
@@ -518,12 +647,17 @@ int ck_ctx_create(int device, ck_ctx** out) {
         ck_ctx_destroy(ctx);
         return fail(CK_EDEVICE, std::string("context setup: ") + hipGetErrorString(e));
     }
-    int lds = 0;
-    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess && lds > 0)
-        ctx->c.lds_max = static_cast<size_t>(lds);
-    else if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
-        ctx->c.lds_max = static_cast<size_t>(lds);
-    const int rc = set_lds_limits(ctx->c.lds_max);
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess && v > 0)
+        ctx->c.lds_block = static_cast<size_t>(v);
+    else if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+        ctx->c.lds_block = static_cast<size_t>(v);
+    ctx->c.lds_cu = ctx->c.lds_block;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess && v > 0)
+        ctx->c.lds_cu = std::max(ctx->c.lds_block, static_cast<size_t>(v));
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0)
+        ctx->c.cus = static_cast<uint32_t>(v);
+    const int rc = set_lds_limits(ctx->c.lds_block);
     if (rc) {
         ck_ctx_destroy(ctx);
         return rc;
@@ -558,16 +692,17 @@ int ck_aes_self_test(ck_ctx* ctx) {
     if (!ctx) return fail(CK_EINVAL, "null ctx");
     CKH(hipSetDevice(ctx->c.device));
     uint32_t* d = nullptr;
-    CKH(hipMalloc(&d, 64 * 4 * sizeof(uint32_t)));
-    hipLaunchKernelGGL(k_aes_kat, dim3(1), dim3(64), 256 * TE_REPL * 4, ctx->c.stream, d);
-    std::vector<uint32_t> h(64 * 4);
+    CKH(hipMalloc(&d, 2 * 64 * 4 * sizeof(uint32_t)));
+    hipLaunchKernelGGL(k_aes_kat, dim3(1), dim3(64), tab_bytes(kRepl), ctx->c.stream, d, kRepl);
+    hipLaunchKernelGGL(k_aes_kat, dim3(1), dim3(64), tab_bytes(0), ctx->c.stream, d + 256, 0u);
+    std::vector<uint32_t> h(2 * 64 * 4);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, h.size() * 4, hipMemcpyDeviceToHost, ctx->c.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->c.stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(CK_EDEVICE, std::string("AES self test: ") + hipGetErrorString(e));
     const uint32_t want[4] = {0x69c4e0d8u, 0x6a7b0430u, 0xd8cdb780u, 0x70b4c55au};
-    for (int lane = 0; lane < 64; ++lane)
+    for (int lane = 0; lane < 128; ++lane)
         if (std::memcmp(h.data() + lane * 4, want, sizeof want))
             return fail(CK_ESELFTEST, "device AES-128 FIPS-197 known-answer test failed");
     return CK_OK;
@@ -583,8 +718,11 @@ static int launch_common(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44],
     CKH(hipSetDevice(ctx->c.device));
     if (stats) std::memset(stats, 0, sizeof *stats);
     std::vector<uint32_t> fails;
+    HostOut out{&fails, probes, tables, nullptr, nullptr, nullptr, nullptr};
     if (count) {
-        rc = run_hbm(ctx->c, *p, rk, first, count, fails, probes, tables, nullptr, nullptr, nullptr, nullptr, stats);
+        const uint32_t tier = choose_tier(ctx->c, *p);
+        rc = tier == CK_TIER_LDS ? run_lds(ctx->c, *p, rk, first, count, out, stats)
+                                 : run_hbm(ctx->c, *p, rk, first, count, out, stats);
         if (rc) return rc;
     }
     std::sort(fails.begin(), fails.end());
@@ -617,9 +755,10 @@ int ck_trial_map(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_
     if (rc) return rc;
     CKH(hipSetDevice(ctx->c.device));
     std::vector<uint32_t> fails;
+    HostOut out{&fails, nullptr, nullptr, key_words, walk, randomness, map};
     const uint32_t saved = ctx->c.chunk_cap;
     ctx->c.chunk_cap = 1;
-    rc = run_hbm(ctx->c, *p, rk, offset, 1, fails, nullptr, nullptr, key_words, walk, randomness, map, nullptr);
+    rc = run_hbm(ctx->c, *p, rk, offset, 1, out, nullptr);
     ctx->c.chunk_cap = saved;
     return rc;
 }

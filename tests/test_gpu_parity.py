@@ -21,6 +21,16 @@ def ctx():
     c.close()
 
 
+TIERS = {"hbm": ck.TIER_HBM, "lds": ck.TIER_LDS}
+
+
+@pytest.fixture(params=["hbm", "lds"])
+def tier(request, ctx):
+    ctx.set_tier(TIERS[request.param])
+    yield request.param
+    ctx.set_tier(ck.TIER_AUTO)
+
+
 def both(mode, pnum, ell, **kw):
     return ck.params(mode, pnum, ell, **kw), ock.parameters(mode, pnum, ell, **kw)
 
@@ -56,7 +66,7 @@ def test_trial_derivation_and_bucket_map(ctx, mode, pnum, ell, kw):
 
 
 @pytest.mark.parametrize("mode", ["bgh25", "drrt18"])
-def test_golden_32768(ctx, fixtures, mode):
+def test_golden_32768(ctx, tier, fixtures, mode):
     g = fixtures["golden_32768"]
     p = ck.params(mode, g["pnum"], g["ell_i"])
     fails = []
@@ -66,7 +76,7 @@ def test_golden_32768(ctx, fixtures, mode):
     assert fails == g["failures"][mode]
 
 
-def test_affine_vectors(ctx, fixtures):
+def test_affine_vectors(ctx, tier, fixtures):
     v = fixtures["affine_vectors"]
     for mode, want in v["failures"].items():
         p = ck.params(mode, v["pnum"], v["ell_i"], input_mode=1)
@@ -74,7 +84,7 @@ def test_affine_vectors(ctx, fixtures):
         assert list(f) == want
 
 
-def test_reference_ranges(ctx, fixtures):
+def test_reference_ranges(ctx, tier, fixtures):
     for row in fixtures["reference_ranges"]:
         p = ck.params(row["mode"], row["pnum"], row["ell_i"], input_mode=row["input_mode"])
         f, _ = ctx.launch(p, ck.batch_key(row["seed"], row["batch"]), 0, row["count"])
@@ -95,10 +105,11 @@ PROBE_SHAPES = [
 
 
 @pytest.mark.parametrize("mode,pnum,ell,kw,count", PROBE_SHAPES)
-def test_failures_probes_tables(ctx, mode, pnum, ell, kw, count):
+def test_failures_probes_tables(ctx, tier, mode, pnum, ell, kw, count):
     p, op = both(mode, pnum, ell, **kw)
     seed, batch, first = 77, 3, 1000
     f, probes, tables, st = ctx.launch_debug(p, ck.batch_key(seed, batch), first, count, tables=True)
+    assert st.tier == TIERS[tier]
     bk = ock.batch_key(seed, batch)
     of, oprobes, ost = ock.run_range(bk, first, count, op, probes=True)
     assert list(f) == list(of)
@@ -110,7 +121,7 @@ def test_failures_probes_tables(ctx, mode, pnum, ell, kw, count):
             assert (tables[i] == bins).all(), i
 
 
-def test_headline_shape_prefix(ctx):
+def test_headline_shape_prefix(ctx, tier):
     # n = 2^14, m = 24576, k = 3, kick 500, ell_i 16, seed 1, batch 0.
     p, op = both("bgh25", 16384, 16, kick=500)
     count = 1024
@@ -120,7 +131,7 @@ def test_headline_shape_prefix(ctx):
     assert (probes == oprobes).all()
 
 
-def test_chunking_does_not_change_results(ctx):
+def test_chunking_does_not_change_results(ctx, tier):
     p, _ = both("drrt18", 16, 8)
     rk = ck.batch_key(123, 3)
     ref, _ = ctx.launch(p, rk, 0, 4096)
@@ -131,3 +142,37 @@ def test_chunking_does_not_change_results(ctx):
     finally:
         ctx.set_chunk(0)
     assert (got == ref).all()
+
+
+def test_direct_build_shapes_reject_lds_tier(ctx):
+    p = ck.params("bgh25", 16, 32)  # half 17: no round cache, HBM tier only
+    ctx.set_tier(ck.TIER_LDS)
+    try:
+        with pytest.raises(ck.CkError):
+            ctx.launch(p, ck.batch_key(1, 0), 0, 4)
+    finally:
+        ctx.set_tier(ck.TIER_AUTO)
+    f, st = ctx.launch(p, ck.batch_key(1, 0), 0, 64)
+    assert st.tier == ck.TIER_HBM
+
+
+@pytest.mark.parametrize("mode,pnum,ell,kw", [("bgh25", 16, 32, {}), ("drrt18", 300, 24, {"input_mode": 1})])
+def test_direct_build_outcomes(ctx, mode, pnum, ell, kw):
+    p, op = both(mode, pnum, ell, **kw)
+    f, probes, _, _ = ctx.launch_debug(p, ck.batch_key(5, 1), 0, 256)
+    of, oprobes, _ = ock.run_range(ock.batch_key(5, 1), 0, 256, op, cached=False, probes=True)
+    assert list(f) == list(of) and (probes == oprobes).all()
+
+
+def test_tiers_agree_on_mixed_shapes(ctx):
+    for mode, pnum, ell, kw in [("bgh25", 500, 16, {"kick": 7, "buckets": 560}),
+                                ("drrt18", 2000, 14, {"input_mode": 1, "kick": 4})]:
+        p = ck.params(mode, pnum, ell, **kw)
+        rk = ck.batch_key(9, 9)
+        res = {}
+        for name, t in TIERS.items():
+            ctx.set_tier(t)
+            res[name] = ctx.launch_debug(p, rk, 17, 1500)
+        ctx.set_tier(ck.TIER_AUTO)
+        assert list(res["hbm"][0]) == list(res["lds"][0])
+        assert (res["hbm"][1] == res["lds"][1]).all()
