@@ -95,11 +95,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--trials", type=int, default=1 << 17, help="trials per step (power of two <= 2^20)")
+    ap.add_argument("--trials", type=int, default=1 << 18, help="trials per step (power of two <= 2^20)")
     ap.add_argument("--tier", choices=["auto", "hbm", "lds"], default="auto")
     ap.add_argument("--chunk", type=int, default=0, help="trials per device pass (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before warmup, copy a buffer of this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -120,6 +122,15 @@ def main():
     ctx.set_tier({"auto": ck.TIER_AUTO, "hbm": ck.TIER_HBM, "lds": ck.TIER_LDS}[args.tier])
     if args.chunk:
         ctx.set_chunk(args.chunk)
+
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib << 18  # float32 elements
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f"copy probe: {args.copy_probe_mib} MiB read + written", file=sys.stderr, flush=True)
 
     keys = {}
 
@@ -150,7 +161,7 @@ def main():
 
     trials = args.steps * t
     value = trials / elapsed
-    chain_launches = max(1, agg["launches"] // 3)
+    chain_launches = max(1, agg["launches"] // 2)  # build + chain per pass
     chain_avg_s = agg["chain_ms"] / 1e3 / chain_launches
     bytes_per_launch = SLOT_BYTES * agg["probes"] / chain_launches
     achieved = bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else 0.0

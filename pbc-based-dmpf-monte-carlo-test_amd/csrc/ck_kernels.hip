@@ -61,10 +61,12 @@ __device__ __forceinline__ void build_cache(const LdsTab& T, const TrialSeed& ts
 
 // build (HBM tier): one wave per trial slot.  CACHED selects the round-cache
 // form (half <= 12); otherwise every bucket evaluation runs AES directly.
-template <int K, bool CACHED>
+// M is the map entry width (u16 when buckets <= 65536); the slot's table
+// (table_bytes, a multiple of 16) is filled with all-ones (empty).
+template <int K, bool CACHED, class M>
 __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
-                                               uint32_t tab_shift, uint32_t* __restrict__ map, size_t map_stride,
-                                               uint32_t* __restrict__ bins, size_t bins_stride,
+                                               uint32_t tab_shift, M* __restrict__ map, size_t map_stride,
+                                               char* __restrict__ tables, size_t table_bytes,
                                                uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
                                                uint32_t* __restrict__ key_out) {
     extern __shared__ uint32_t lds[];
@@ -82,36 +84,36 @@ __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t f
         key_out[slot * 44 + lane] = v;
     }
 
-    uint32_t* m = map + slot * map_stride;
+    M* m = map + slot * map_stride;
     if constexpr (CACHED) {
         uint16_t* cache = reinterpret_cast<uint16_t*>(lds + (256u << tab_shift)) + (size_t(wave) << (S.half + 3));
         build_cache(T, ts, S, cache, lane);
         for (uint32_t o = lane; o < S.pnum; o += 64) {
             const uint64_t el = element_of(S, ts.shift, ts.step, o);
 #pragma unroll
-            for (int w = 0; w < K; ++w) m[o * K + w] = feistel_cached(cache, S, el, K, w);
+            for (int w = 0; w < K; ++w) m[o * K + w] = static_cast<M>(feistel_cached(cache, S, el, K, w));
         }
     } else {
         for (uint32_t o = lane; o < S.pnum; o += 64) {
             const uint64_t el = element_of(S, ts.shift, ts.step, o);
-            for (int w = 0; w < K; ++w) m[o * K + w] = feistel_direct(T, ts.rk, S, el, K, w);
+            for (int w = 0; w < K; ++w) m[o * K + w] = static_cast<M>(feistel_direct(T, ts.rk, S, el, K, w));
         }
     }
 
-    uint4* t4 = reinterpret_cast<uint4*>(bins + slot * bins_stride);
+    uint4* t4 = reinterpret_cast<uint4*>(tables + slot * table_bytes);
     const uint4 e4 = make_uint4(EMPTY, EMPTY, EMPTY, EMPTY);
-    for (uint32_t i = lane; i < bins_stride / 4; i += 64) t4[i] = e4;
+    for (uint32_t i = lane; i < table_bytes / 16; i += 64) t4[i] = e4;
     if (lane == 0) {
         walks[slot] = ts.walk;
         rnds[slot] = ts.rnd;
     }
 }
 
-// chain (HBM tier): one lane per trial slot.
-template <int K, bool BGH>
+// chain (HBM tier): one lane per trial slot; map and table widths M and E.
+template <int K, bool BGH, class M, class E>
 __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t count,
-                                               const uint32_t* __restrict__ map, size_t map_stride,
-                                               uint32_t* __restrict__ bins, size_t bins_stride,
+                                               const M* __restrict__ map, size_t map_stride,
+                                               char* __restrict__ tables, size_t table_bytes,
                                                const uint64_t* __restrict__ walks, const uint64_t* __restrict__ rnds,
                                                uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
                                                uint32_t* __restrict__ probes_out) {
@@ -119,8 +121,8 @@ __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t
     bool failed = false;
     uint32_t pr = 0;
     if (slot < count) {
-        const uint32_t* m = map + slot * map_stride;
-        uint32_t* t = bins + slot * bins_stride;
+        const M* m = map + slot * map_stride;
+        E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
         if constexpr (BGH)
             failed = chain_bgh<K>(S, m, t, walks[slot], rnds[slot], pr);
         else
@@ -226,12 +228,15 @@ struct Workspace {
 
 struct Ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;        // chains (HBM tier) and fused kernels (LDS tier)
+    hipStream_t build_stream = nullptr;  // builds of the next HBM pass, overlapping the chains
     hipEvent_t ev[4] = {};
+    hipEvent_t ev_build[2][2] = {};  // [buffer][start, end]
+    hipEvent_t ev_chain[2][2] = {};
     Workspace ws;
     uint32_t tier = CK_TIER_AUTO;
     uint32_t chunk_cap = 0;
-    size_t budget = size_t(48) << 30;  // HBM budget for per-slot maps and tables
+    size_t budget = size_t(48) << 30;  // HBM budget for the double-buffered maps and tables
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
     uint32_t cus = 1;
@@ -268,6 +273,13 @@ static int check_params(const ck_params* p) {
 static constexpr uint32_t kMaxCacheHalf = 12;
 static constexpr uint32_t kRepl = 5;  // T0 copies = 2^5 for conflict-free lookups
 static size_t tab_bytes(uint32_t shift) { return size_t(256) << shift << 2; }
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Entry widths: a u16 map holds bucket indices < 65536; a u16 table entry holds
+// ordinal | choice << 14, which needs ordinals < 2^14 and never equals the
+// all-ones empty entry (choice 3 with ordinal 16383 would).
+static bool map16_ok(const ck_params& p) { return p.buckets <= 65536; }
+static bool table16_ok(const ck_params& p) { return p.pnum < 16384 || (p.pnum == 16384 && p.k <= 3); }
 
 struct BuildPlan {
     bool cached;
@@ -298,8 +310,8 @@ static LdsPlan plan_lds(const ck_params& p, size_t lds_block, size_t lds_cu) {
     LdsPlan L;
     if (p.half_bits > kMaxCacheHalf) return L;
     auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
-    L.map16 = p.buckets <= 65536;
-    L.bins16 = p.pnum < 16384 || (p.pnum == 16384 && p.k <= 3);
+    L.map16 = map16_ok(p);
+    L.bins16 = table16_ok(p);
     L.cache_b = a16((size_t(8) << p.half_bits) * 2);
     L.map_b = a16(size_t(p.pnum) * p.k * (L.map16 ? 2 : 4));
     L.bins_b = a16(size_t(p.buckets) * (L.bins16 ? 2 : 4));
@@ -324,54 +336,80 @@ static LdsPlan plan_lds(const ck_params& p, size_t lds_block, size_t lds_cu) {
     return L;
 }
 
-// Shapes whose chains stay on chip with at least this many resident trials
-// per CU use the LDS tier by default; larger tables go to HBM.
-static constexpr uint32_t kAutoLdsMinWaves = 4;
-
-static uint32_t choose_tier(const Ctx& c, const ck_params& p) {
-    if (c.tier != CK_TIER_AUTO) return c.tier;
-    const LdsPlan L = plan_lds(p, c.lds_block, c.lds_cu);
-    return L.ok && L.blocks_per_cu * L.waves >= kAutoLdsMinWaves ? CK_TIER_LDS : CK_TIER_HBM;
+// The HBM tier is faster than the LDS tier on every measured shape (n = 1024 and
+// n = 2^14, both modes): an LDS chain probe costs two dependent LDS round
+// trips on one lane while the HBM tier keeps ~10^5 chains in flight.  Auto
+// therefore picks HBM; the LDS tier stays selectable.
+static uint32_t choose_tier(const Ctx& c, const ck_params&) {
+    return c.tier != CK_TIER_AUTO ? c.tier : CK_TIER_HBM;
 }
 
-struct Layout {
-    size_t map_stride, bins_stride;  // u32 entries per slot
-    size_t off_cnt, off_map, off_bins, off_walk, off_rnd, off_probes, off_fail, off_key, off_tables, total;
+// HBM-tier pass buffer: [counters | map C x map_stride M | tables C x table_bytes |
+// walk C | rnd C | probes C | fail C | keys C x 44], twice (double-buffered).
+struct HbmPlan {
+    bool map16, table16;
+    size_t map_stride;   // map entries per slot
+    size_t map_bytes;    // per slot
+    size_t table_bytes;  // per slot
+    size_t off_cnt, off_map, off_tables, off_walk, off_rnd, off_probes, off_fail, off_key, buffer;
 };
-static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-static Layout layout_for(const ck_params& p, uint32_t chunk, bool hbm, bool tables) {
-    Layout L{};
-    L.map_stride = hbm ? (size_t(p.pnum) * p.k + 63) & ~size_t(63) : 0;
-    L.bins_stride = hbm ? (size_t(p.buckets) + 63) & ~size_t(63) : 0;
+static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk) {
+    HbmPlan h{};
+    h.map16 = map16_ok(p);
+    h.table16 = table16_ok(p);
+    const size_t mw = h.map16 ? 2 : 4, tw = h.table16 ? 2 : 4;
+    h.map_bytes = align256(size_t(p.pnum) * p.k * mw);
+    h.map_stride = h.map_bytes / mw;
+    h.table_bytes = align256(size_t(p.buckets) * tw);
+    size_t o = 0;
+    h.off_cnt = o;
+    o = align256(o + CNT_WORDS * 4);
+    h.off_map = o;
+    o = align256(o + h.map_bytes * chunk);
+    h.off_tables = o;
+    o = align256(o + h.table_bytes * chunk);
+    h.off_walk = o;
+    o = align256(o + size_t(chunk) * 8);
+    h.off_rnd = o;
+    o = align256(o + size_t(chunk) * 8);
+    h.off_probes = o;
+    o = align256(o + size_t(chunk) * 4);
+    h.off_fail = o;
+    o = align256(o + size_t(chunk) * 4);
+    h.off_key = o;
+    o = align256(o + size_t(chunk) * 44 * 4);
+    h.buffer = o;
+    return h;
+}
+
+// Trials per pass: 65536 chains in flight saturate the random-access path
+// (measured: 32768 slower, 131072 and more no faster).
+static constexpr uint32_t kHbmChunk = 1u << 16;
+static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count) {
+    const HbmPlan one = plan_hbm(p, 1);
+    const size_t per_slot = one.map_bytes + one.table_bytes + 8 + 8 + 4 + 4 + 176;
+    const size_t fit = std::max<size_t>(1, c.budget / (2 * per_slot));
+    const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
+    return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>({fit, size_t(target), size_t(count)})));
+}
+
+// LDS-tier per-launch scratch: [counters | probes C | fail C | tables C x buckets].
+struct LdsScratch {
+    size_t off_cnt, off_probes, off_fail, off_tables, total;
+};
+static LdsScratch lds_scratch(const ck_params& p, uint32_t chunk, bool tables) {
+    LdsScratch L{};
     size_t o = 0;
     L.off_cnt = o;
     o = align256(o + CNT_WORDS * 4);
-    L.off_map = o;
-    o = align256(o + L.map_stride * chunk * 4);
-    L.off_bins = o;
-    o = align256(o + L.bins_stride * chunk * 4);
-    L.off_walk = o;
-    o = align256(o + (hbm ? size_t(chunk) * 8 : 0));
-    L.off_rnd = o;
-    o = align256(o + (hbm ? size_t(chunk) * 8 : 0));
     L.off_probes = o;
     o = align256(o + size_t(chunk) * 4);
     L.off_fail = o;
     o = align256(o + size_t(chunk) * 4);
-    L.off_key = o;
-    o = align256(o + (hbm ? size_t(chunk) * 44 * 4 : 0));
     L.off_tables = o;
-    o = align256(o + (tables && !hbm ? size_t(chunk) * p.buckets * 4 : 0));
+    o = align256(o + (tables ? size_t(chunk) * p.buckets * 4 : 0));
     L.total = o;
     return L;
-}
-
-static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count) {
-    const size_t per_slot = ((size_t(p.pnum) * p.k + 63) & ~size_t(63)) * 4 +
-                            ((size_t(p.buckets) + 63) & ~size_t(63)) * 4 + 16 + 8 + 176;
-    const size_t fit = std::max<size_t>(1, c.budget / per_slot);
-    const uint32_t target = c.chunk_cap ? c.chunk_cap : (1u << 17);  // ~2048 waves in flight
-    return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>({fit, size_t(target), size_t(count)})));
 }
 static uint32_t lds_chunk(const Ctx& c, uint32_t count, bool tables, const ck_params& p) {
     uint32_t target = c.chunk_cap ? c.chunk_cap : (1u << 20);
@@ -389,30 +427,65 @@ static int ensure_ws(Ctx& c, size_t bytes) {
     return CK_OK;
 }
 
-template <int K>
-static void launch_build(const BuildPlan& bp, const Shape& S, const RoundKeys& bk, uint32_t first, uint32_t count,
-                         uint32_t* map, size_t map_stride, uint32_t* bins, size_t bins_stride, uint64_t* walks,
-                         uint64_t* rnds, uint32_t* keys, hipStream_t st) {
-    const uint32_t blocks = (count + bp.waves - 1) / bp.waves;
+struct PassArgs {
+    Shape S;
+    RoundKeys bk;
+    uint32_t first, count;
+    char* map;
+    size_t map_stride;
+    char* tables;
+    size_t table_bytes;
+    uint64_t* walks;
+    uint64_t* rnds;
+    uint32_t* cnt;
+    uint32_t* fail;
+    uint32_t* probes;  // null unless requested
+    uint32_t* keys;    // null unless requested
+};
+
+template <int K, class M>
+static void launch_build_t(const BuildPlan& bp, const PassArgs& a, hipStream_t st) {
+    const uint32_t blocks = (a.count + bp.waves - 1) / bp.waves;
+    M* map = reinterpret_cast<M*>(a.map);
     if (bp.cached)
-        hipLaunchKernelGGL((k_build<K, true>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, S, bk, first, count,
-                           kRepl, map, map_stride, bins, bins_stride, walks, rnds, keys);
+        hipLaunchKernelGGL((k_build<K, true, M>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, a.S, a.bk, a.first,
+                           a.count, kRepl, map, a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
     else
-        hipLaunchKernelGGL((k_build<K, false>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, S, bk, first, count,
-                           kRepl, map, map_stride, bins, bins_stride, walks, rnds, keys);
+        hipLaunchKernelGGL((k_build<K, false, M>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, a.S, a.bk, a.first,
+                           a.count, kRepl, map, a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
+}
+template <int K>
+static void launch_build_k(const BuildPlan& bp, bool map16, const PassArgs& a, hipStream_t st) {
+    if (map16)
+        launch_build_t<K, uint16_t>(bp, a, st);
+    else
+        launch_build_t<K, uint32_t>(bp, a, st);
 }
 
-template <int K>
-static void launch_chain(bool bgh, const Shape& S, uint32_t first, uint32_t count, const uint32_t* map,
-                         size_t map_stride, uint32_t* bins, size_t bins_stride, const uint64_t* walks,
-                         const uint64_t* rnds, uint32_t* cnt, uint32_t* fo, uint32_t* probes, hipStream_t st) {
-    const uint32_t blocks = (count + 255) / 256;
-    if (bgh)
-        hipLaunchKernelGGL((k_chain<K, true>), dim3(blocks), dim3(256), 0, st, S, first, count, map, map_stride, bins,
-                           bins_stride, walks, rnds, cnt, fo, probes);
+template <int K, bool BGH, class M, class E>
+static void launch_chain_t(const PassArgs& a, hipStream_t st) {
+    const uint32_t blocks = (a.count + 255) / 256;
+    hipLaunchKernelGGL((k_chain<K, BGH, M, E>), dim3(blocks), dim3(256), 0, st, a.S, a.first, a.count,
+                       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds,
+                       a.cnt, a.fail, a.probes);
+}
+template <int K, bool BGH>
+static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+    if (h.map16 && h.table16)
+        launch_chain_t<K, BGH, uint16_t, uint16_t>(a, st);
+    else if (h.map16)
+        launch_chain_t<K, BGH, uint16_t, uint32_t>(a, st);
+    else if (h.table16)
+        launch_chain_t<K, BGH, uint32_t, uint16_t>(a, st);
     else
-        hipLaunchKernelGGL((k_chain<K, false>), dim3(blocks), dim3(256), 0, st, S, first, count, map, map_stride, bins,
-                           bins_stride, walks, rnds, cnt, fo, probes);
+        launch_chain_t<K, BGH, uint32_t, uint32_t>(a, st);
+}
+template <int K>
+static void launch_chain_k(bool bgh, const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+    if (bgh)
+        launch_chain_kb<K, true>(h, a, st);
+    else
+        launch_chain_kb<K, false>(h, a, st);
 }
 
 template <int K, bool BGH>
@@ -464,27 +537,26 @@ static int set_lds_limits(size_t lds_block) {
         return CK_OK;
     };
     int rc;
-    for (const void* f : {reinterpret_cast<const void*>(&k_build<2, true>), reinterpret_cast<const void*>(&k_build<2, false>),
-                          reinterpret_cast<const void*>(&k_build<3, true>), reinterpret_cast<const void*>(&k_build<3, false>),
-                          reinterpret_cast<const void*>(&k_build<4, true>), reinterpret_cast<const void*>(&k_build<4, false>)})
+#define B1(K, C) reinterpret_cast<const void*>(&k_build<K, C, uint16_t>), reinterpret_cast<const void*>(&k_build<K, C, uint32_t>)
+    for (const void* f : {B1(2, true), B1(2, false), B1(3, true), B1(3, false), B1(4, true), B1(4, false)})
         if ((rc = opt(f))) return rc;
+#undef B1
     return for_each_lds_kernel(opt);
 }
 
 struct HostOut {
     std::vector<uint32_t>* fails;
     uint32_t* probes;   // per trial, may be null
-    uint32_t* tables;   // per trial x buckets, may be null
+    uint32_t* tables;   // per trial x buckets (ordinal | choice << 30), may be null
     uint32_t* keys;     // HBM tier debug: per trial 44 words
     uint64_t* walks;
     uint64_t* rnds;
     uint32_t* map;      // HBM tier debug: per trial pnum*k
 };
 
-static int collect(Ctx& c, const uint32_t* cnt, const uint32_t* fo, uint32_t n, HostOut& out, ck_stats* st) {
+static int take_counters(const uint32_t* cnt, const uint32_t* fo, uint32_t n, HostOut& out, ck_stats* st) {
     uint32_t counters[4] = {0, 0, 0, 0};
-    CKH(hipMemcpyAsync(counters, cnt, sizeof counters, hipMemcpyDeviceToHost, c.stream));
-    CKH(hipStreamSynchronize(c.stream));
+    CKH(hipMemcpy(counters, cnt, sizeof counters, hipMemcpyDeviceToHost));
     const uint32_t nf = counters[CNT_FAIL];
     if (nf > n) return fail(CK_ERANGE, "device failure count out of range");
     if (nf) {
@@ -496,75 +568,113 @@ static int collect(Ctx& c, const uint32_t* cnt, const uint32_t* fo, uint32_t n, 
     return CK_OK;
 }
 
-// HBM tier: build + chain per chunk of trial slots.
+// Copies n rows of `width` entries (u16 or u32, row pitch `pitch` bytes) into
+// u32 rows of `width` entries, widening table entries to ordinal | choice << 30.
+static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width, uint32_t n, bool src16, bool entries) {
+    if (!src16) {
+        CKH(hipMemcpy2D(dst, width * 4, src, pitch, width * 4, n, hipMemcpyDeviceToHost));
+        return CK_OK;
+    }
+    std::vector<uint16_t> tmp(width * n);
+    CKH(hipMemcpy2D(tmp.data(), width * 2, src, pitch, width * 2, n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < tmp.size(); ++i) {
+        const uint32_t v = tmp[i];
+        dst[i] = !entries ? v : v == 0xffffu ? EMPTY : ((v & 0x3fffu) | ((v >> 14) << 30));
+    }
+    return CK_OK;
+}
+
+// HBM tier: build + chain per pass of up to `chunk` trial slots.  Two buffers
+// and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
                    ck_stats* st) {
-    const Shape S = make_shape(p);
-    RoundKeys bk;
-    std::memcpy(bk.w, rk, sizeof bk.w);
     const uint32_t chunk = hbm_chunk(c, p, count);
-    const Layout L = layout_for(p, chunk, true, false);
-    int rc = ensure_ws(c, L.total);
+    const HbmPlan h = plan_hbm(p, chunk);
+    int rc = ensure_ws(c, 2 * h.buffer);
     if (rc) return rc;
-    char* base = static_cast<char*>(c.ws.base);
-    auto* cnt = reinterpret_cast<uint32_t*>(base + L.off_cnt);
-    auto* map = reinterpret_cast<uint32_t*>(base + L.off_map);
-    auto* bins = reinterpret_cast<uint32_t*>(base + L.off_bins);
-    auto* walks = reinterpret_cast<uint64_t*>(base + L.off_walk);
-    auto* rnds = reinterpret_cast<uint64_t*>(base + L.off_rnd);
-    auto* probes = reinterpret_cast<uint32_t*>(base + L.off_probes);
-    auto* fo = reinterpret_cast<uint32_t*>(base + L.off_fail);
-    auto* keys = reinterpret_cast<uint32_t*>(base + L.off_key);
     const BuildPlan bp = plan_build(p, c.lds_block);
     const bool bgh = p.mode == CK_MODE_BGH25;
+    PassArgs args[2];
+    for (int b = 0; b < 2; ++b) {
+        char* base = static_cast<char*>(c.ws.base) + b * h.buffer;
+        PassArgs& a = args[b];
+        a.S = make_shape(p);
+        std::memcpy(a.bk.w, rk, sizeof a.bk.w);
+        a.map = base + h.off_map;
+        a.map_stride = h.map_stride;
+        a.tables = base + h.off_tables;
+        a.table_bytes = h.table_bytes;
+        a.walks = reinterpret_cast<uint64_t*>(base + h.off_walk);
+        a.rnds = reinterpret_cast<uint64_t*>(base + h.off_rnd);
+        a.cnt = reinterpret_cast<uint32_t*>(base + h.off_cnt);
+        a.fail = reinterpret_cast<uint32_t*>(base + h.off_fail);
+        a.probes = out.probes ? reinterpret_cast<uint32_t*>(base + h.off_probes) : nullptr;
+        a.keys = out.keys ? reinterpret_cast<uint32_t*>(base + h.off_key) : nullptr;
+    }
     if (st) {
         st->tier = CK_TIER_HBM;
         st->chunk = chunk;
     }
-    for (uint32_t done = 0; done < count;) {
-        const uint32_t n = std::min(chunk, count - done);
-        const uint32_t at = first + done;
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c.stream, cnt, CNT_WORDS);
-        CKH(hipEventRecord(c.ev[0], c.stream));
-        uint32_t* kout = out.keys ? keys : nullptr;
+    const uint32_t passes = (count + chunk - 1) / chunk;
+    auto enqueue = [&](uint32_t i) -> int {
+        const int b = i & 1;
+        PassArgs& a = args[b];
+        a.first = first + i * chunk;
+        a.count = std::min(chunk, count - i * chunk);
+        if (i >= 2) CKH(hipStreamWaitEvent(c.build_stream, c.ev_chain[b][1], 0));
+        CKH(hipEventRecord(c.ev_build[b][0], c.build_stream));
         switch (p.k) {
-            case 2: launch_build<2>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, kout, c.stream); break;
-            case 3: launch_build<3>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, kout, c.stream); break;
-            default: launch_build<4>(bp, S, bk, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, kout, c.stream); break;
+            case 2: launch_build_k<2>(bp, h.map16, a, c.build_stream); break;
+            case 3: launch_build_k<3>(bp, h.map16, a, c.build_stream); break;
+            default: launch_build_k<4>(bp, h.map16, a, c.build_stream); break;
         }
         CKH(hipGetLastError());
-        CKH(hipEventRecord(c.ev[1], c.stream));
-        uint32_t* pout = out.probes ? probes : nullptr;
+        CKH(hipEventRecord(c.ev_build[b][1], c.build_stream));
+        CKH(hipStreamWaitEvent(c.stream, c.ev_build[b][1], 0));
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c.stream, a.cnt, CNT_WORDS);
+        CKH(hipEventRecord(c.ev_chain[b][0], c.stream));
         switch (p.k) {
-            case 2: launch_chain<2>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, cnt, fo, pout, c.stream); break;
-            case 3: launch_chain<3>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, cnt, fo, pout, c.stream); break;
-            default: launch_chain<4>(bgh, S, at, n, map, L.map_stride, bins, L.bins_stride, walks, rnds, cnt, fo, pout, c.stream); break;
+            case 2: launch_chain_k<2>(bgh, h, a, c.stream); break;
+            case 3: launch_chain_k<3>(bgh, h, a, c.stream); break;
+            default: launch_chain_k<4>(bgh, h, a, c.stream); break;
         }
         CKH(hipGetLastError());
-        CKH(hipEventRecord(c.ev[2], c.stream));
-        rc = collect(c, cnt, fo, n, out, st);
-        if (rc) return rc;
-        if (out.probes) CKH(hipMemcpy(out.probes + done, probes, size_t(n) * 4, hipMemcpyDeviceToHost));
-        if (out.tables)
-            CKH(hipMemcpy2D(out.tables + size_t(done) * p.buckets, size_t(p.buckets) * 4, bins, L.bins_stride * 4,
-                            size_t(p.buckets) * 4, n, hipMemcpyDeviceToHost));
-        if (out.keys) CKH(hipMemcpy(out.keys + size_t(done) * 44, keys, size_t(n) * 44 * 4, hipMemcpyDeviceToHost));
-        if (out.walks) CKH(hipMemcpy(out.walks + done, walks, size_t(n) * 8, hipMemcpyDeviceToHost));
-        if (out.rnds) CKH(hipMemcpy(out.rnds + done, rnds, size_t(n) * 8, hipMemcpyDeviceToHost));
-        if (out.map)
-            CKH(hipMemcpy2D(out.map + size_t(done) * p.pnum * p.k, size_t(p.pnum) * p.k * 4, map, L.map_stride * 4,
-                            size_t(p.pnum) * p.k * 4, n, hipMemcpyDeviceToHost));
+        CKH(hipEventRecord(c.ev_chain[b][1], c.stream));
+        return CK_OK;
+    };
+    auto gather = [&](uint32_t i) -> int {
+        const int b = i & 1;
+        const PassArgs& a = args[b];
+        CKH(hipEventSynchronize(c.ev_chain[b][1]));
+        int r = take_counters(a.cnt, a.fail, a.count, out, st);
+        if (r) return r;
+        const size_t done = size_t(i) * chunk;
+        if (out.probes) CKH(hipMemcpy(out.probes + done, a.probes, size_t(a.count) * 4, hipMemcpyDeviceToHost));
+        if (out.tables &&
+            (r = copy_rows(out.tables + done * p.buckets, a.tables, h.table_bytes, p.buckets, a.count, h.table16, true)))
+            return r;
+        if (out.map && (r = copy_rows(out.map + done * p.pnum * p.k, a.map, h.map_bytes, size_t(p.pnum) * p.k,
+                                      a.count, h.map16, false)))
+            return r;
+        if (out.keys) CKH(hipMemcpy(out.keys + done * 44, a.keys, size_t(a.count) * 44 * 4, hipMemcpyDeviceToHost));
+        if (out.walks) CKH(hipMemcpy(out.walks + done, a.walks, size_t(a.count) * 8, hipMemcpyDeviceToHost));
+        if (out.rnds) CKH(hipMemcpy(out.rnds + done, a.rnds, size_t(a.count) * 8, hipMemcpyDeviceToHost));
         if (st) {
-            float b = 0, ch = 0;
-            CKH(hipEventElapsedTime(&b, c.ev[0], c.ev[1]));
-            CKH(hipEventElapsedTime(&ch, c.ev[1], c.ev[2]));
-            st->build_ms += b;
-            st->chain_ms += ch;
+            float bms = 0, cms = 0;
+            CKH(hipEventElapsedTime(&bms, c.ev_build[b][0], c.ev_build[b][1]));
+            CKH(hipEventElapsedTime(&cms, c.ev_chain[b][0], c.ev_chain[b][1]));
+            st->build_ms += bms;
+            st->chain_ms += cms;
             st->launches += 2;
         }
-        done += n;
+        return CK_OK;
+    };
+    if ((rc = enqueue(0))) return rc;
+    for (uint32_t i = 1; i < passes; ++i) {
+        if ((rc = enqueue(i))) return rc;
+        if ((rc = gather(i - 1))) return rc;
     }
-    return CK_OK;
+    return gather(passes - 1);
 }
 
 // LDS tier: one fused persistent launch per chunk.
@@ -576,7 +686,7 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     RoundKeys bk;
     std::memcpy(bk.w, rk, sizeof bk.w);
     const uint32_t chunk = lds_chunk(c, count, out.tables != nullptr, p);
-    const Layout L = layout_for(p, chunk, false, out.tables != nullptr);
+    const LdsScratch L = lds_scratch(p, chunk, out.tables != nullptr);
     int rc = ensure_ws(c, L.total);
     if (rc) return rc;
     char* base = static_cast<char*>(c.ws.base);
@@ -604,7 +714,8 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         }
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev[1], c.stream));
-        rc = collect(c, cnt, fo, n, out, st);
+        CKH(hipStreamSynchronize(c.stream));
+        rc = take_counters(cnt, fo, n, out, st);
         if (rc) return rc;
         if (out.probes) CKH(hipMemcpy(out.probes + done, probes, size_t(n) * 4, hipMemcpyDeviceToHost));
         if (out.tables)
@@ -641,7 +752,13 @@ int ck_ctx_create(int device, ck_ctx** out) {
     auto* ctx = new ck_ctx();
     ctx->c.device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->c.build_stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->c.ev[i]);
+    for (int b = 0; b < 2 && e == hipSuccess; ++b)
+        for (int j = 0; j < 2 && e == hipSuccess; ++j) {
+            e = hipEventCreate(&ctx->c.ev_build[b][j]);
+            if (e == hipSuccess) e = hipEventCreate(&ctx->c.ev_chain[b][j]);
+        }
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_te0), te0(), 256 * sizeof(uint32_t));
     if (e != hipSuccess) {
         ck_ctx_destroy(ctx);
@@ -672,6 +789,12 @@ void ck_ctx_destroy(ck_ctx* ctx) {
     if (ctx->c.ws.base) (void)hipFree(ctx->c.ws.base);
     for (auto& ev : ctx->c.ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (int b = 0; b < 2; ++b)
+        for (int j = 0; j < 2; ++j) {
+            if (ctx->c.ev_build[b][j]) (void)hipEventDestroy(ctx->c.ev_build[b][j]);
+            if (ctx->c.ev_chain[b][j]) (void)hipEventDestroy(ctx->c.ev_chain[b][j]);
+        }
+    if (ctx->c.build_stream) (void)hipStreamDestroy(ctx->c.build_stream);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
 }
