@@ -50,6 +50,13 @@ enum {
     CK_TIER_LDS = 2   /* trials per workgroup, table and bucket map in LDS */
 };
 
+/* HBM-tier table layout. */
+enum {
+    CK_LAYOUT_AUTO = 0, /* descriptor when it fits and tables are not requested, else map */
+    CK_LAYOUT_MAP = 1,  /* entry = ordinal | choice; candidates looked up in a bucket map */
+    CK_LAYOUT_DESC = 2  /* entry = the occupant's packed candidate buckets | choice (k*bits+2 <= 63) */
+};
+
 typedef struct ck_stats {
     uint64_t trials;      /* trials completed by the call */
     uint64_t failed;      /* failing trials */
@@ -85,6 +92,8 @@ void ck_ctx_destroy(ck_ctx* ctx);
 int ck_ctx_set_tier(ck_ctx* ctx, uint32_t tier);
 /* Caps the trials per device pass (0 = library default). */
 int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
+/* Forces an HBM-tier table layout (CK_LAYOUT_*). */
+int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
 
 /* FIPS-197 AES-128 known answer through the device AES path
  * (aes_self_test / cached_aes_self_test, cuckoo_bench.cu:332-354). */
@@ -104,6 +113,15 @@ int ck_launch(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], ui
 int ck_launch_debug(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t first,
                     uint32_t count, uint32_t* failed_offsets, uint32_t* n_failed, uint32_t* probes,
                     uint32_t* tables, ck_stats* stats);
+
+/* Test hook: like ck_launch, plus per-trial probe counts and an
+ * order-independent digest of every trial's final table: the wrapping sum over
+ * occupied buckets b of mix64(b*0x9E3779B97F4A7C15 ^ H ^ choice<<61), with
+ * H = sum over w<k of mix64(w<<32 | bucket(occupant, w)) and mix64 the
+ * SplitMix64 output function.  Works for every tier and layout. */
+int ck_launch_digest(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t first,
+                     uint32_t count, uint32_t* failed_offsets, uint32_t* n_failed, uint32_t* probes,
+                     uint64_t* digests, ck_stats* stats);
 
 /* Test hook: the device key derivation and bucket map of one trial:
  * words[0..43] = PRP key schedule, walk/randomness of the first BGH draw,

@@ -203,3 +203,31 @@ def ref_shim():
     lib.ref_trial.argtypes = [C.c_int, C.c_int, U32, U32, U64, U64, U32, C.c_int, P(U32)]
     lib.ref_trial.restype = C.c_int
     return lib
+
+
+# ----------------------------------------------------------- table digest
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def table_digest(bins: np.ndarray, bmap: np.ndarray, k: int) -> int:
+    """Order-independent digest of a final table (entry = ordinal | choice << 30):
+    the wrapping sum over occupied buckets b of mix64(b*golden ^ H ^ choice << 61),
+    H = sum over w < k of mix64(w << 32 | bucket(occupant, w))."""
+    with np.errstate(over="ignore"):
+        b = np.nonzero(bins != 0xFFFFFFFF)[0].astype(np.uint64)
+        e = bins[b.astype(np.int64)].astype(np.uint64)
+        ordinal = (e & np.uint64(0x3FFFFFFF)).astype(np.int64)
+        choice = e >> np.uint64(30)
+        h = np.zeros(len(b), dtype=np.uint64)
+        for w in range(k):
+            cand = bmap[ordinal * k + w].astype(np.uint64)
+            h += _mix64((np.uint64(w) << np.uint64(32)) | cand)
+        terms = _mix64((b * _GOLD) ^ h ^ (choice << np.uint64(61)))
+        return int(np.sum(terms, dtype=np.uint64))

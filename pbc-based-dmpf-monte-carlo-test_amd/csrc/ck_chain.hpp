@@ -157,4 +157,145 @@ __device__ __forceinline__ bool chain_drrt(const Shape& S, const M* __restrict__
     return failed;
 }
 
+// ---------------------------------------------------------------------------
+// Descriptor layout (HBM tier, when k*bb + 2 <= 63 with bb = bit width of the
+// largest bucket index): a table entry holds the occupant's k bucket indices
+// and its current choice, desc | choice << (k*bb), and all-ones is empty.  An
+// eviction therefore reads the occupant's candidates from the slot it just
+// loaded instead of from the bucket map; new items take their descriptors from
+// a per-trial stream in insertion order, prefetched one item ahead.
+struct DescCodec {
+    uint32_t bb;        // bits per bucket index
+    uint32_t cs;        // choice shift = k*bb
+    uint64_t bmask;     // (1 << bb) - 1
+    uint64_t dmask;     // (1 << cs) - 1
+    __device__ __forceinline__ uint32_t bucket(uint64_t d, uint32_t w) const {
+        return static_cast<uint32_t>((d >> (w * bb)) & bmask);
+    }
+};
+constexpr uint64_t EMPTY64 = ~0ull;
+
+template <int K>
+__device__ __forceinline__ bool chain_bgh_desc(const Shape& S, const DescCodec& D, const uint64_t* __restrict__ desc,
+                                               uint64_t* __restrict__ t, uint64_t walk, uint64_t rnd,
+                                               uint32_t& probes) {
+    uint32_t item = 0, coll = 0, stash_left = S.stash, pr = 0;
+    uint64_t cur = desc[0];
+    uint64_t next = S.pnum > 1 ? desc[1] : 0;
+    bool failed = false;
+    while (item < S.pnum) {
+        if (rnd < K) rnd = splitmix(walk);
+        const uint32_t which = take_choice<K>(rnd);
+        const uint32_t b = D.bucket(cur, which);
+        const uint64_t old = t[b];
+        ++pr;
+        bool advance = old == EMPTY64;
+        if (!advance && ++coll > S.kick) {
+            if (!stash_left) {
+                failed = true;
+                break;
+            }
+            --stash_left;  // the item in hand goes to the stash; nothing is written
+            advance = true;
+        } else {
+            t[b] = cur | (uint64_t(which) << D.cs);
+        }
+        if (advance) {
+            ++item;
+            cur = next;
+            if (item + 1 < S.pnum) next = desc[item + 1];
+            coll = 0;
+        } else {
+            cur = old & D.dmask;
+        }
+    }
+    probes = pr;
+    return failed;
+}
+
+template <int K>
+__device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec& D, const uint64_t* __restrict__ desc,
+                                                uint64_t* __restrict__ t, uint32_t& probes) {
+    uint32_t pr = 0, stash_left = S.stash;
+    bool failed = false;
+    uint64_t next = desc[0];
+    for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
+        const uint64_t d = next;
+        if (item + 1 < S.pnum) next = desc[item + 1];
+        bool placed = false;
+#pragma unroll
+        for (int w = 0; w < K; ++w) {
+            if (placed) break;
+            ++pr;
+            const uint32_t b = D.bucket(d, w);
+            if (t[b] == EMPTY64) {
+                t[b] = d | (uint64_t(w) << D.cs);
+                placed = true;
+            }
+        }
+        if (placed) continue;
+        uint64_t cur = d;
+        uint32_t choice = 0, target = D.bucket(d, 0), kicks = 0;
+        for (;;) {
+            const uint64_t disp = t[target];
+            ++pr;
+            if (disp == EMPTY64) {
+                t[target] = cur | (uint64_t(choice) << D.cs);
+                break;
+            }
+            if (kicks >= S.kick) {
+                if (stash_left) {
+                    --stash_left;
+                    break;
+                }
+                failed = true;
+                break;
+            }
+            t[target] = cur | (uint64_t(choice) << D.cs);
+            cur = disp & D.dmask;
+            choice = static_cast<uint32_t>(disp >> D.cs) & 3u;
+            const uint32_t prev = target;
+            bool found = false;
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+                if (found) break;
+                choice = (choice + 1 == K) ? 0 : choice + 1;
+                const uint32_t nb = D.bucket(cur, choice);
+                if (nb != prev) {
+                    target = nb;
+                    found = true;
+                }
+            }
+            if (!found) {
+                if (stash_left) {
+                    --stash_left;
+                    break;
+                }
+                failed = true;
+                break;
+            }
+            ++kicks;
+        }
+    }
+    probes = pr;
+    return failed;
+}
+
+// Order-independent digest of a final table: the wrapping sum over occupied
+// buckets b of mix64(b * golden ^ H ^ choice << 61), where H is the wrapping
+// sum over w < k of mix64(w << 32 | bucket(occupant, w)).  mix64 is the
+// SplitMix64 output function.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// Hash of an item's candidate list, independent of how it is stored.
+__host__ __device__ __forceinline__ uint64_t cand_hash_term(uint32_t w, uint32_t bucket) {
+    return mix64((uint64_t(w) << 32) | bucket);
+}
+__host__ __device__ __forceinline__ uint64_t slot_digest(uint32_t b, uint64_t cand_hash, uint32_t choice) {
+    return mix64(uint64_t(b) * 0x9E3779B97F4A7C15ull ^ cand_hash ^ (uint64_t(choice) << 61));
+}
+
 }  // namespace ck

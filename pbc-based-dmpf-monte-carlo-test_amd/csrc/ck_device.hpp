@@ -17,6 +17,8 @@ struct Shape {
     uint32_t k, pnum, buckets, kick, stash, half, input_mode, mode;
     uint64_t bucket_size, domain, mask, range_mask;
     uint64_t div_magic;  // ceil(2^64 / bucket_size) for the 32-bit quotient; 0 when bucket_size == 1
+    uint32_t dbits;      // bits per bucket index in a packed candidate descriptor
+    uint32_t pad;
 };
 struct RoundKeys {
     uint32_t w[44];

@@ -59,9 +59,24 @@ __device__ __forceinline__ void build_cache(const LdsTab& T, const TrialSeed& ts
     wave_lds_sync();
 }
 
+// The bucket map row of ordinal o: k entries, or one packed descriptor.
+template <int K, class M>
+__device__ __forceinline__ void store_candidates(M* m, const Shape& S, uint32_t o, const uint32_t (&c)[K]) {
+    if constexpr (sizeof(M) == 8) {
+        uint64_t d = 0;
+#pragma unroll
+        for (int w = 0; w < K; ++w) d |= uint64_t(c[w]) << (w * S.dbits);
+        m[o] = d;
+    } else {
+#pragma unroll
+        for (int w = 0; w < K; ++w) m[o * K + w] = static_cast<M>(c[w]);
+    }
+}
+
 // build (HBM tier): one wave per trial slot.  CACHED selects the round-cache
 // form (half <= 12); otherwise every bucket evaluation runs AES directly.
-// M is the map entry width (u16 when buckets <= 65536); the slot's table
+// M is the map entry width (u16 when buckets <= 65536), or u64 for the
+// descriptor layout (one packed candidate list per ordinal); the slot's table
 // (table_bytes, a multiple of 16) is filled with all-ones (empty).
 template <int K, bool CACHED, class M>
 __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
@@ -90,13 +105,17 @@ __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t f
         build_cache(T, ts, S, cache, lane);
         for (uint32_t o = lane; o < S.pnum; o += 64) {
             const uint64_t el = element_of(S, ts.shift, ts.step, o);
+            uint32_t c[K];
 #pragma unroll
-            for (int w = 0; w < K; ++w) m[o * K + w] = static_cast<M>(feistel_cached(cache, S, el, K, w));
+            for (int w = 0; w < K; ++w) c[w] = feistel_cached(cache, S, el, K, w);
+            store_candidates<K>(m, S, o, c);
         }
     } else {
         for (uint32_t o = lane; o < S.pnum; o += 64) {
             const uint64_t el = element_of(S, ts.shift, ts.step, o);
-            for (int w = 0; w < K; ++w) m[o * K + w] = static_cast<M>(feistel_direct(T, ts.rk, S, el, K, w));
+            uint32_t c[K];
+            for (int w = 0; w < K; ++w) c[w] = feistel_direct(T, ts.rk, S, el, K, w);
+            store_candidates<K>(m, S, o, c);
         }
     }
 
@@ -123,14 +142,54 @@ __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t
     if (slot < count) {
         const M* m = map + slot * map_stride;
         E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
-        if constexpr (BGH)
-            failed = chain_bgh<K>(S, m, t, walks[slot], rnds[slot], pr);
-        else
-            failed = chain_drrt<K>(S, m, t, pr);
+        if constexpr (sizeof(M) == 8) {
+            const DescCodec D{S.dbits, K * S.dbits, (1ull << S.dbits) - 1, (1ull << (K * S.dbits)) - 1};
+            if constexpr (BGH)
+                failed = chain_bgh_desc<K>(S, D, m, t, walks[slot], rnds[slot], pr);
+            else
+                failed = chain_drrt_desc<K>(S, D, m, t, pr);
+        } else {
+            if constexpr (BGH)
+                failed = chain_bgh<K>(S, m, t, walks[slot], rnds[slot], pr);
+            else
+                failed = chain_drrt<K>(S, m, t, pr);
+        }
         if (probes_out) probes_out[slot] = pr;
     }
     append_failures(failed, first + slot, counters + CNT_FAIL, fail_offsets);
     add_wave_total(pr, reinterpret_cast<unsigned long long*>(counters + CNT_PROBES));
+}
+
+// Debug: per-slot digest of the final table (slot_digest over occupied buckets).
+template <int K, class M, class E>
+__global__ __launch_bounds__(256) void k_digest(Shape S, uint32_t count, const M* __restrict__ map, size_t map_stride,
+                                                const char* __restrict__ tables, size_t table_bytes,
+                                                unsigned long long* __restrict__ out) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    const M* m = map + slot * map_stride;
+    const E* t = reinterpret_cast<const E*>(tables + slot * table_bytes);
+    uint64_t sum = 0;
+    for (uint32_t b = 0; b < S.buckets; ++b) {
+        const E e = t[b];
+        uint64_t h = 0;
+        uint32_t choice;
+        if constexpr (sizeof(E) == 8) {
+            if (e == EMPTY64) continue;
+            const uint32_t cs = K * S.dbits;
+            choice = static_cast<uint32_t>(e >> cs) & 3u;
+#pragma unroll
+            for (int w = 0; w < K; ++w) h += cand_hash_term(w, static_cast<uint32_t>((e >> (w * S.dbits)) & ((1ull << S.dbits) - 1)));
+        } else {
+            if (e == empty_entry<E>()) continue;
+            const uint32_t o = entry_ordinal<E>(e);
+            choice = entry_choice<E>(e);
+#pragma unroll
+            for (int w = 0; w < K; ++w) h += cand_hash_term(w, m[o * K + w]);
+        }
+        sum += slot_digest(b, h, choice);
+    }
+    out[slot] = sum;
 }
 
 // LDS tier: a persistent wave per trial at a time.  Wave region layout:
@@ -139,7 +198,8 @@ template <int K, bool BGH, class M, class E>
 __global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t first, uint32_t count, uint32_t tab_shift,
                                              uint32_t wave_bytes, uint32_t cache_bytes, uint32_t map_bytes,
                                              uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
-                                             uint32_t* __restrict__ probes_out, uint32_t* __restrict__ tables_out) {
+                                             uint32_t* __restrict__ probes_out, uint32_t* __restrict__ tables_out,
+                                             unsigned long long* __restrict__ digests_out) {
     extern __shared__ uint32_t lds[];
     const LdsTab T = stage_tab(lds, tab_shift);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -184,6 +244,21 @@ __global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t fir
                 const E e = table[i];
                 out[i] = e == empty_entry<E>() ? EMPTY : (entry_ordinal<E>(e) | (entry_choice<E>(e) << 30));
             }
+        }
+        if (digests_out) {
+            uint64_t part = 0;
+            for (uint32_t b = lane; b < S.buckets; b += 64) {
+                const E e = table[b];
+                if (e == empty_entry<E>()) continue;
+                const uint32_t o = entry_ordinal<E>(e);
+                uint64_t h = 0;
+#pragma unroll
+                for (int w = 0; w < K; ++w) h += cand_hash_term(w, map[o * K + w]);
+                part += slot_digest(b, h, entry_choice<E>(e));
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d);
+            digests_out[slot] = part;  // same value from every lane
         }
         if (probes_out) probes_out[slot] = pr;  // same value from every lane
         if (failed) {
@@ -235,6 +310,7 @@ struct Ctx {
     hipEvent_t ev_chain[2][2] = {};
     Workspace ws;
     uint32_t tier = CK_TIER_AUTO;
+    uint32_t layout = CK_LAYOUT_AUTO;
     uint32_t chunk_cap = 0;
     size_t budget = size_t(48) << 30;  // HBM budget for the double-buffered maps and tables
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
@@ -257,6 +333,9 @@ static Shape make_shape(const ck_params& p) {
     s.mask = p.mask;
     s.range_mask = p.range_mask;
     s.div_magic = p.bucket_size <= 1 ? 0 : (~0ull / p.bucket_size) + 1;
+    uint32_t bb = 1;
+    while (bb < 32 && (uint64_t(1) << bb) < p.buckets) ++bb;
+    s.dbits = bb;
     return s;
 }
 
@@ -344,21 +423,28 @@ static uint32_t choose_tier(const Ctx& c, const ck_params&) {
     return c.tier != CK_TIER_AUTO ? c.tier : CK_TIER_HBM;
 }
 
-// HBM-tier pass buffer: [counters | map C x map_stride M | tables C x table_bytes |
-// walk C | rnd C | probes C | fail C | keys C x 44], twice (double-buffered).
+// Descriptor layout fits when k packed bucket indices and a 2-bit choice leave
+// the top bit of a u64 clear (so all-ones stays the empty entry).
+static bool desc_ok(const ck_params& p) { return p.k * make_shape(p).dbits + 2 <= 63; }
+
+// HBM-tier pass buffer: [counters | map C x map_bytes | tables C x table_bytes |
+// walk C | rnd C | probes C | fail C | keys C x 44 | digests C], twice
+// (double-buffered).  In the descriptor layout the "map" is one u64 candidate
+// descriptor per ordinal and the table entries are u64.
 struct HbmPlan {
-    bool map16, table16;
+    bool desc, map16, table16;
     size_t map_stride;   // map entries per slot
     size_t map_bytes;    // per slot
     size_t table_bytes;  // per slot
-    size_t off_cnt, off_map, off_tables, off_walk, off_rnd, off_probes, off_fail, off_key, buffer;
+    size_t off_cnt, off_map, off_tables, off_walk, off_rnd, off_probes, off_fail, off_key, off_digest, buffer;
 };
-static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk) {
+static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, bool desc) {
     HbmPlan h{};
-    h.map16 = map16_ok(p);
-    h.table16 = table16_ok(p);
-    const size_t mw = h.map16 ? 2 : 4, tw = h.table16 ? 2 : 4;
-    h.map_bytes = align256(size_t(p.pnum) * p.k * mw);
+    h.desc = desc;
+    h.map16 = !desc && map16_ok(p);
+    h.table16 = !desc && table16_ok(p);
+    const size_t mw = desc ? 8 : h.map16 ? 2 : 4, tw = desc ? 8 : h.table16 ? 2 : 4;
+    h.map_bytes = align256(size_t(p.pnum) * (desc ? 1 : p.k) * mw);
     h.map_stride = h.map_bytes / mw;
     h.table_bytes = align256(size_t(p.buckets) * tw);
     size_t o = 0;
@@ -378,6 +464,8 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk) {
     o = align256(o + size_t(chunk) * 4);
     h.off_key = o;
     o = align256(o + size_t(chunk) * 44 * 4);
+    h.off_digest = o;
+    o = align256(o + size_t(chunk) * 8);
     h.buffer = o;
     return h;
 }
@@ -385,17 +473,17 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk) {
 // Trials per pass: 65536 chains in flight saturate the random-access path
 // (measured: 32768 slower, 131072 and more no faster).
 static constexpr uint32_t kHbmChunk = 1u << 16;
-static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count) {
-    const HbmPlan one = plan_hbm(p, 1);
+static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, bool desc) {
+    const HbmPlan one = plan_hbm(p, 1, desc);
     const size_t per_slot = one.map_bytes + one.table_bytes + 8 + 8 + 4 + 4 + 176;
     const size_t fit = std::max<size_t>(1, c.budget / (2 * per_slot));
     const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
     return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>({fit, size_t(target), size_t(count)})));
 }
 
-// LDS-tier per-launch scratch: [counters | probes C | fail C | tables C x buckets].
+// LDS-tier per-launch scratch: [counters | probes C | fail C | digests C | tables C x buckets].
 struct LdsScratch {
-    size_t off_cnt, off_probes, off_fail, off_tables, total;
+    size_t off_cnt, off_probes, off_fail, off_digest, off_tables, total;
 };
 static LdsScratch lds_scratch(const ck_params& p, uint32_t chunk, bool tables) {
     LdsScratch L{};
@@ -406,6 +494,8 @@ static LdsScratch lds_scratch(const ck_params& p, uint32_t chunk, bool tables) {
     o = align256(o + size_t(chunk) * 4);
     L.off_fail = o;
     o = align256(o + size_t(chunk) * 4);
+    L.off_digest = o;
+    o = align256(o + size_t(chunk) * 8);
     L.off_tables = o;
     o = align256(o + (tables ? size_t(chunk) * p.buckets * 4 : 0));
     L.total = o;
@@ -441,6 +531,7 @@ struct PassArgs {
     uint32_t* fail;
     uint32_t* probes;  // null unless requested
     uint32_t* keys;    // null unless requested
+    unsigned long long* digests;  // null unless requested
 };
 
 template <int K, class M>
@@ -455,8 +546,10 @@ static void launch_build_t(const BuildPlan& bp, const PassArgs& a, hipStream_t s
                            a.count, kRepl, map, a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
 }
 template <int K>
-static void launch_build_k(const BuildPlan& bp, bool map16, const PassArgs& a, hipStream_t st) {
-    if (map16)
+static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+    if (h.desc)
+        launch_build_t<K, uint64_t>(bp, a, st);
+    else if (h.map16)
         launch_build_t<K, uint16_t>(bp, a, st);
     else
         launch_build_t<K, uint32_t>(bp, a, st);
@@ -469,9 +562,30 @@ static void launch_chain_t(const PassArgs& a, hipStream_t st) {
                        reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds,
                        a.cnt, a.fail, a.probes);
 }
+template <int K, class M, class E>
+static void launch_digest_t(const PassArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k_digest<K, M, E>), dim3((a.count + 255) / 256), dim3(256), 0, st, a.S, a.count,
+                       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.digests);
+}
+template <int K>
+static void launch_digest_k(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+    if (h.desc)
+        launch_digest_t<K, uint64_t, uint64_t>(a, st);
+    else if (h.map16 && h.table16)
+        launch_digest_t<K, uint16_t, uint16_t>(a, st);
+    else if (h.map16)
+        launch_digest_t<K, uint16_t, uint32_t>(a, st);
+    else if (h.table16)
+        launch_digest_t<K, uint32_t, uint16_t>(a, st);
+    else
+        launch_digest_t<K, uint32_t, uint32_t>(a, st);
+}
+
 template <int K, bool BGH>
 static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
-    if (h.map16 && h.table16)
+    if (h.desc)
+        launch_chain_t<K, BGH, uint64_t, uint64_t>(a, st);
+    else if (h.map16 && h.table16)
         launch_chain_t<K, BGH, uint16_t, uint16_t>(a, st);
     else if (h.map16)
         launch_chain_t<K, BGH, uint16_t, uint32_t>(a, st);
@@ -491,11 +605,11 @@ static void launch_chain_k(bool bgh, const HbmPlan& h, const PassArgs& a, hipStr
 template <int K, bool BGH>
 static void launch_lds_kb(const LdsPlan& L, uint32_t blocks, const Shape& S, const RoundKeys& bk, uint32_t first,
                           uint32_t count, uint32_t* cnt, uint32_t* fo, uint32_t* probes, uint32_t* tables,
-                          hipStream_t st) {
+                          unsigned long long* digests, hipStream_t st) {
     const dim3 grid(blocks), block(64 * L.waves);
     const uint32_t wb = static_cast<uint32_t>(L.wave_b), cb = static_cast<uint32_t>(L.cache_b),
                    mb = static_cast<uint32_t>(L.map_b);
-#define LDS_ARGS S, bk, first, count, L.tab_shift, wb, cb, mb, cnt, fo, probes, tables
+#define LDS_ARGS S, bk, first, count, L.tab_shift, wb, cb, mb, cnt, fo, probes, tables, digests
     if (L.map16 && L.bins16)
         hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint16_t>), grid, block, L.lds, st, LDS_ARGS);
     else if (L.map16)
@@ -509,11 +623,11 @@ static void launch_lds_kb(const LdsPlan& L, uint32_t blocks, const Shape& S, con
 template <int K>
 static void launch_lds(bool bgh, const LdsPlan& L, uint32_t blocks, const Shape& S, const RoundKeys& bk,
                        uint32_t first, uint32_t count, uint32_t* cnt, uint32_t* fo, uint32_t* probes,
-                       uint32_t* tables, hipStream_t st) {
+                       uint32_t* tables, unsigned long long* digests, hipStream_t st) {
     if (bgh)
-        launch_lds_kb<K, true>(L, blocks, S, bk, first, count, cnt, fo, probes, tables, st);
+        launch_lds_kb<K, true>(L, blocks, S, bk, first, count, cnt, fo, probes, tables, digests, st);
     else
-        launch_lds_kb<K, false>(L, blocks, S, bk, first, count, cnt, fo, probes, tables, st);
+        launch_lds_kb<K, false>(L, blocks, S, bk, first, count, cnt, fo, probes, tables, digests, st);
 }
 
 template <class Fn>
@@ -537,7 +651,9 @@ static int set_lds_limits(size_t lds_block) {
         return CK_OK;
     };
     int rc;
-#define B1(K, C) reinterpret_cast<const void*>(&k_build<K, C, uint16_t>), reinterpret_cast<const void*>(&k_build<K, C, uint32_t>)
+#define B1(K, C)                                                                                          \
+    reinterpret_cast<const void*>(&k_build<K, C, uint16_t>), reinterpret_cast<const void*>(&k_build<K, C, uint32_t>), \
+        reinterpret_cast<const void*>(&k_build<K, C, uint64_t>)
     for (const void* f : {B1(2, true), B1(2, false), B1(3, true), B1(3, false), B1(4, true), B1(4, false)})
         if ((rc = opt(f))) return rc;
 #undef B1
@@ -552,6 +668,7 @@ struct HostOut {
     uint64_t* walks;
     uint64_t* rnds;
     uint32_t* map;      // HBM tier debug: per trial pnum*k
+    uint64_t* digests;  // per trial table digest, may be null
 };
 
 static int take_counters(const uint32_t* cnt, const uint32_t* fo, uint32_t n, HostOut& out, ck_stats* st) {
@@ -588,8 +705,11 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
 // and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
                    ck_stats* st) {
-    const uint32_t chunk = hbm_chunk(c, p, count);
-    const HbmPlan h = plan_hbm(p, chunk);
+    const bool desc = c.layout == CK_LAYOUT_DESC || (c.layout == CK_LAYOUT_AUTO && !out.tables && desc_ok(p));
+    if (desc && !desc_ok(p)) return fail(CK_EINVAL, "descriptor layout needs k*bits(buckets-1) + 2 <= 63");
+    if (desc && out.tables) return fail(CK_EINVAL, "the descriptor layout does not keep ordinals; use the map layout for tables");
+    const uint32_t chunk = hbm_chunk(c, p, count, desc);
+    const HbmPlan h = plan_hbm(p, chunk, desc);
     int rc = ensure_ws(c, 2 * h.buffer);
     if (rc) return rc;
     const BuildPlan bp = plan_build(p, c.lds_block);
@@ -610,6 +730,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         a.fail = reinterpret_cast<uint32_t*>(base + h.off_fail);
         a.probes = out.probes ? reinterpret_cast<uint32_t*>(base + h.off_probes) : nullptr;
         a.keys = out.keys ? reinterpret_cast<uint32_t*>(base + h.off_key) : nullptr;
+        a.digests = out.digests ? reinterpret_cast<unsigned long long*>(base + h.off_digest) : nullptr;
     }
     if (st) {
         st->tier = CK_TIER_HBM;
@@ -624,9 +745,9 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         if (i >= 2) CKH(hipStreamWaitEvent(c.build_stream, c.ev_chain[b][1], 0));
         CKH(hipEventRecord(c.ev_build[b][0], c.build_stream));
         switch (p.k) {
-            case 2: launch_build_k<2>(bp, h.map16, a, c.build_stream); break;
-            case 3: launch_build_k<3>(bp, h.map16, a, c.build_stream); break;
-            default: launch_build_k<4>(bp, h.map16, a, c.build_stream); break;
+            case 2: launch_build_k<2>(bp, h, a, c.build_stream); break;
+            case 3: launch_build_k<3>(bp, h, a, c.build_stream); break;
+            default: launch_build_k<4>(bp, h, a, c.build_stream); break;
         }
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev_build[b][1], c.build_stream));
@@ -640,12 +761,21 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         }
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev_chain[b][1], c.stream));
+        if (a.digests) {
+            switch (p.k) {
+                case 2: launch_digest_k<2>(h, a, c.stream); break;
+                case 3: launch_digest_k<3>(h, a, c.stream); break;
+                default: launch_digest_k<4>(h, a, c.stream); break;
+            }
+            CKH(hipGetLastError());
+        }
         return CK_OK;
     };
     auto gather = [&](uint32_t i) -> int {
         const int b = i & 1;
         const PassArgs& a = args[b];
         CKH(hipEventSynchronize(c.ev_chain[b][1]));
+        if (a.digests) CKH(hipStreamSynchronize(c.stream));
         int r = take_counters(a.cnt, a.fail, a.count, out, st);
         if (r) return r;
         const size_t done = size_t(i) * chunk;
@@ -653,9 +783,21 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         if (out.tables &&
             (r = copy_rows(out.tables + done * p.buckets, a.tables, h.table_bytes, p.buckets, a.count, h.table16, true)))
             return r;
-        if (out.map && (r = copy_rows(out.map + done * p.pnum * p.k, a.map, h.map_bytes, size_t(p.pnum) * p.k,
-                                      a.count, h.map16, false)))
+        if (out.map && !h.desc &&
+            (r = copy_rows(out.map + done * p.pnum * p.k, a.map, h.map_bytes, size_t(p.pnum) * p.k, a.count, h.map16,
+                           false)))
             return r;
+        if (out.map && h.desc) {
+            std::vector<uint64_t> d(size_t(p.pnum) * a.count);
+            CKH(hipMemcpy2D(d.data(), size_t(p.pnum) * 8, a.map, h.map_bytes, size_t(p.pnum) * 8, a.count,
+                            hipMemcpyDeviceToHost));
+            const uint32_t bb = a.S.dbits;
+            for (size_t i = 0; i < d.size(); ++i)
+                for (uint32_t w = 0; w < p.k; ++w)
+                    out.map[done * p.pnum * p.k + i * p.k + w] = static_cast<uint32_t>((d[i] >> (w * bb)) & ((1ull << bb) - 1));
+        }
+        if (out.digests)
+            CKH(hipMemcpy(out.digests + done, a.digests, size_t(a.count) * 8, hipMemcpyDeviceToHost));
         if (out.keys) CKH(hipMemcpy(out.keys + done * 44, a.keys, size_t(a.count) * 44 * 4, hipMemcpyDeviceToHost));
         if (out.walks) CKH(hipMemcpy(out.walks + done, a.walks, size_t(a.count) * 8, hipMemcpyDeviceToHost));
         if (out.rnds) CKH(hipMemcpy(out.rnds + done, a.rnds, size_t(a.count) * 8, hipMemcpyDeviceToHost));
@@ -694,6 +836,7 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     auto* probes = reinterpret_cast<uint32_t*>(base + L.off_probes);
     auto* fo = reinterpret_cast<uint32_t*>(base + L.off_fail);
     auto* tables = reinterpret_cast<uint32_t*>(base + L.off_tables);
+    auto* digests = reinterpret_cast<unsigned long long*>(base + L.off_digest);
     const bool bgh = p.mode == CK_MODE_BGH25;
     const uint32_t resident = plan.blocks_per_cu * c.cus;
     if (st) {
@@ -707,10 +850,11 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         CKH(hipEventRecord(c.ev[0], c.stream));
         uint32_t* pout = out.probes ? probes : nullptr;
         uint32_t* tout = out.tables ? tables : nullptr;
+        unsigned long long* dout = out.digests ? digests : nullptr;
         switch (p.k) {
-            case 2: launch_lds<2>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, c.stream); break;
-            case 3: launch_lds<3>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, c.stream); break;
-            default: launch_lds<4>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, c.stream); break;
+            case 2: launch_lds<2>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, dout, c.stream); break;
+            case 3: launch_lds<3>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, dout, c.stream); break;
+            default: launch_lds<4>(bgh, plan, blocks, S, bk, first + done, n, cnt, fo, pout, tout, dout, c.stream); break;
         }
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev[1], c.stream));
@@ -721,6 +865,7 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         if (out.tables)
             CKH(hipMemcpy(out.tables + size_t(done) * p.buckets, tables, size_t(n) * p.buckets * 4,
                           hipMemcpyDeviceToHost));
+        if (out.digests) CKH(hipMemcpy(out.digests + done, digests, size_t(n) * 8, hipMemcpyDeviceToHost));
         if (st) {
             float f = 0;
             CKH(hipEventElapsedTime(&f, c.ev[0], c.ev[1]));
@@ -833,7 +978,7 @@ int ck_aes_self_test(ck_ctx* ctx) {
 
 static int launch_common(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t first, uint32_t count,
                          uint32_t* failed_offsets, uint32_t* n_failed, uint32_t* probes, uint32_t* tables,
-                         ck_stats* stats) {
+                         uint64_t* digests, ck_stats* stats) {
     if (!ctx || !rk || !n_failed || (count && !failed_offsets)) return fail(CK_EINVAL, "null argument");
     int rc = check_params(p);
     if (rc) return rc;
@@ -841,7 +986,7 @@ static int launch_common(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44],
     CKH(hipSetDevice(ctx->c.device));
     if (stats) std::memset(stats, 0, sizeof *stats);
     std::vector<uint32_t> fails;
-    HostOut out{&fails, probes, tables, nullptr, nullptr, nullptr, nullptr};
+    HostOut out{&fails, probes, tables, nullptr, nullptr, nullptr, nullptr, digests};
     if (count) {
         const uint32_t tier = choose_tier(ctx->c, *p);
         rc = tier == CK_TIER_LDS ? run_lds(ctx->c, *p, rk, first, count, out, stats)
@@ -862,13 +1007,26 @@ static int launch_common(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44],
 
 int ck_launch(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t first, uint32_t count,
               uint32_t* failed_offsets, uint32_t* n_failed, ck_stats* stats) {
-    return launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, nullptr, nullptr, stats);
+    return launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, nullptr, nullptr, nullptr, stats);
 }
 
 int ck_launch_debug(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t first, uint32_t count,
                     uint32_t* failed_offsets, uint32_t* n_failed, uint32_t* probes, uint32_t* tables,
                     ck_stats* stats) {
-    return launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, probes, tables, stats);
+    return launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, probes, tables, nullptr, stats);
+}
+
+int ck_launch_digest(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t first, uint32_t count,
+                     uint32_t* failed_offsets, uint32_t* n_failed, uint32_t* probes, uint64_t* digests,
+                     ck_stats* stats) {
+    if (!digests) return fail(CK_EINVAL, "null digests");
+    return launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, probes, nullptr, digests, stats);
+}
+
+int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout) {
+    if (!ctx || layout > CK_LAYOUT_DESC) return fail(CK_EINVAL, "bad layout");
+    ctx->c.layout = layout;
+    return CK_OK;
 }
 
 int ck_trial_map(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t offset, uint32_t key_words[44],
@@ -878,7 +1036,7 @@ int ck_trial_map(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_
     if (rc) return rc;
     CKH(hipSetDevice(ctx->c.device));
     std::vector<uint32_t> fails;
-    HostOut out{&fails, nullptr, nullptr, key_words, walk, randomness, map};
+    HostOut out{&fails, nullptr, nullptr, key_words, walk, randomness, map, nullptr};
     const uint32_t saved = ctx->c.chunk_cap;
     ctx->c.chunk_cap = 1;
     rc = run_hbm(ctx->c, *p, rk, offset, 1, out, nullptr);

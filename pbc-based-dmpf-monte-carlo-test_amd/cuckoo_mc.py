@@ -18,6 +18,7 @@ CK_OK = 0
 BGH25, DRRT18 = 1, 2
 DERIVED = 0xFFFFFFFF
 TIER_AUTO, TIER_HBM, TIER_LDS = 0, 1, 2
+LAYOUT_AUTO, LAYOUT_MAP, LAYOUT_DESC = 0, 1, 2
 MODES = {"bgh25": BGH25, "drrt18": DRRT18}
 
 
@@ -57,6 +58,9 @@ def _load():
         "ck_ctx_destroy": (None, [vp]),
         "ck_ctx_set_tier": (C.c_int, [vp, U32]),
         "ck_ctx_set_chunk": (C.c_int, [vp, U32]),
+        "ck_ctx_set_layout": (C.c_int, [vp, U32]),
+        "ck_launch_digest": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U64),
+                                       P(Stats)]),
         "ck_aes_self_test": (C.c_int, [vp]),
         "ck_launch": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(Stats)]),
         "ck_launch_debug": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U32),
@@ -75,8 +79,8 @@ LIB = _load()
 
 # Every symbol declared in include/cuckoo_mc.h.
 EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "ck_ctx_destroy",
-           "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_aes_self_test", "ck_launch", "ck_launch_debug",
-           "ck_trial_map", "ck_last_error"]
+           "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_ctx_set_layout", "ck_aes_self_test", "ck_launch",
+           "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error"]
 
 
 def _check(rc):
@@ -139,6 +143,9 @@ class Context:
     def set_chunk(self, trials: int):
         _check(LIB.ck_ctx_set_chunk(self.h, trials))
 
+    def set_layout(self, layout: int):
+        _check(LIB.ck_ctx_set_layout(self.h, layout))
+
     def self_test(self):
         _check(LIB.ck_aes_self_test(self.h))
 
@@ -162,6 +169,18 @@ class Context:
                                    _u32p(probes), _u32p(tab) if tab is not None else None, C.byref(st)))
         return (out[:n.value].copy(), probes[:count].copy(),
                 tab.reshape(count, p.buckets) if tab is not None else None, st)
+
+    def launch_digest(self, p: Params, rk: np.ndarray, first: int, count: int):
+        """Failing offsets, per-trial probe counts and per-trial table digests."""
+        rk = np.ascontiguousarray(rk, dtype=np.uint32)
+        out = np.zeros(max(count, 1), dtype=np.uint32)
+        probes = np.zeros(max(count, 1), dtype=np.uint32)
+        dig = np.zeros(max(count, 1), dtype=np.uint64)
+        n = C.c_uint32()
+        st = Stats()
+        _check(LIB.ck_launch_digest(self.h, C.byref(p), _u32p(rk), first, count, _u32p(out), C.byref(n),
+                                    _u32p(probes), dig.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(st)))
+        return out[:n.value].copy(), probes[:count].copy(), dig[:count].copy(), st
 
     def trial_map(self, p: Params, rk: np.ndarray, offset: int):
         rk = np.ascontiguousarray(rk, dtype=np.uint32)

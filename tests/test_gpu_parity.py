@@ -176,3 +176,77 @@ def test_tiers_agree_on_mixed_shapes(ctx):
         ctx.set_tier(ck.TIER_AUTO)
         assert list(res["hbm"][0]) == list(res["lds"][0])
         assert (res["hbm"][1] == res["lds"][1]).all()
+
+
+# ------------------------------------------- layouts and table digests
+
+COMBOS = {"hbm-map": (ck.TIER_HBM, ck.LAYOUT_MAP), "hbm-desc": (ck.TIER_HBM, ck.LAYOUT_DESC),
+          "lds": (ck.TIER_LDS, ck.LAYOUT_AUTO)}
+
+DIGEST_SHAPES = [
+    ("bgh25", 16, 8, {}, 1024),
+    ("drrt18", 16, 8, {"input_mode": 1}, 1024),
+    ("bgh25", 300, 12, {"k": 2, "buckets": 420, "stash": 2, "kick": 30}, 512),
+    ("bgh25", 300, 12, {"k": 4, "buckets": 320, "stash": 1, "kick": 10}, 512),
+    ("drrt18", 300, 12, {"k": 2, "buckets": 420, "stash": 3, "kick": 10}, 512),
+    ("drrt18", 300, 12, {"k": 4, "buckets": 310, "stash": 1, "kick": 5}, 512),
+    ("bgh25", 1024, 16, {"kick": 500}, 512),
+    ("drrt18", 2000, 14, {"input_mode": 1, "kick": 4}, 256),
+]
+
+
+@pytest.mark.parametrize("combo", list(COMBOS))
+@pytest.mark.parametrize("mode,pnum,ell,kw,count", DIGEST_SHAPES)
+def test_table_digests(ctx, combo, mode, pnum, ell, kw, count):
+    tier, layout = COMBOS[combo]
+    ctx.set_tier(tier)
+    ctx.set_layout(layout)
+    try:
+        p, op = both(mode, pnum, ell, **kw)
+        seed, batch, first = 31, 4, 77
+        f, probes, dig, st = ctx.launch_digest(p, ck.batch_key(seed, batch), first, count)
+    finally:
+        ctx.set_tier(ck.TIER_AUTO)
+        ctx.set_layout(ck.LAYOUT_AUTO)
+    bk = ock.batch_key(seed, batch)
+    of, oprobes, _ = ock.run_range(bk, first, count, op, probes=True)
+    assert list(f) == list(of)
+    assert (probes == oprobes).all()
+    for i in range(count):
+        _, bins, _ = ock.trial_run(bk, first + i, op)
+        assert int(dig[i]) == ock.table_digest(bins, ock.bucket_map(bk, first + i, op), op.k), i
+
+
+def test_headline_shape_descriptor_layout(ctx):
+    p, op = both("bgh25", 16384, 16, kick=500)
+    ctx.set_layout(ck.LAYOUT_DESC)
+    try:
+        f, probes, dig, st = ctx.launch_digest(p, ck.batch_key(1, 0), 0, 128)
+    finally:
+        ctx.set_layout(ck.LAYOUT_AUTO)
+    bk = ock.batch_key(1, 0)
+    of, oprobes, _ = ock.run_range(bk, 0, 128, op, probes=True)
+    assert list(f) == list(of) and (probes == oprobes).all()
+    for i in range(0, 128, 9):
+        _, bins, _ = ock.trial_run(bk, i, op)
+        assert int(dig[i]) == ock.table_digest(bins, ock.bucket_map(bk, i, op), 3)
+
+
+def test_descriptor_trial_map(ctx):
+    p, op = both("drrt18", 4096, 16, input_mode=1)
+    ctx.set_layout(ck.LAYOUT_DESC)
+    try:
+        key, walk, rnd, m = ctx.trial_map(p, ck.batch_key(8, 2), 99)
+    finally:
+        ctx.set_layout(ck.LAYOUT_AUTO)
+    assert (m == ock.bucket_map(ock.batch_key(8, 2), 99, op)).all()
+
+
+def test_descriptor_layout_rejects_oversized_shapes(ctx):
+    p = ck.params("bgh25", 1 << 20, 20, buckets=1363148, kick=1000, stash=2)  # 3 x 21 bits + 2 > 63
+    ctx.set_layout(ck.LAYOUT_DESC)
+    try:
+        with pytest.raises(ck.CkError):
+            ctx.launch(p, ck.batch_key(4, 0), 0, 1)
+    finally:
+        ctx.set_layout(ck.LAYOUT_AUTO)
