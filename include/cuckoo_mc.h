@@ -50,11 +50,13 @@ enum {
     CK_TIER_LDS = 2   /* trials per workgroup, table and bucket map in LDS */
 };
 
-/* HBM-tier table layout. */
+/* HBM-tier table layout and probe form. */
 enum {
-    CK_LAYOUT_AUTO = 0, /* descriptor when it fits and tables are not requested, else map */
-    CK_LAYOUT_MAP = 1,  /* entry = ordinal | choice; candidates looked up in a bucket map */
-    CK_LAYOUT_DESC = 2  /* entry = the occupant's packed candidate buckets | choice (k*bits+2 <= 63) */
+    CK_LAYOUT_AUTO = 0,       /* DESC when it fits and tables are not requested, else MAP_XCHG */
+    CK_LAYOUT_MAP = 1,        /* entry = ordinal | choice (u16 when it fits); load + store probes */
+    CK_LAYOUT_DESC = 2,       /* entry = occupant's packed candidates | choice (k*bits+2 <= 63); exchange probes */
+    CK_LAYOUT_MAP_XCHG = 3,   /* u32 entry = ordinal | choice; exchange probes */
+    CK_LAYOUT_DESC_PLAIN = 4  /* descriptor entries; load + store probes */
 };
 
 typedef struct ck_stats {

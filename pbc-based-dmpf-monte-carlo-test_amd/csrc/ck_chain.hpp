@@ -281,6 +281,154 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
     return failed;
 }
 
+// ---------------------------------------------------------------------------
+// One-round-trip probes.  On this GPU family vector-memory loads and stores
+// retire in order under one counter, so a probe written as load + store makes
+// the next probe's load wait for the store's acknowledgement as well: two memory
+// round trips per probe.  Here every probe is a single read-modify-write at the
+// L2 -- an exchange that installs the item and returns the occupant, or, for a
+// drrt18 first-placement attempt, a compare-and-swap against empty.  Where the
+// reference would have left the slot untouched (the kick bound trips), the
+// occupant is written back, so tables match the load/store form exactly.
+template <class E>
+__device__ __forceinline__ E xchg(E* p, E v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class E>
+__device__ __forceinline__ E cas_empty(E* p, E v) {
+    E expected = static_cast<E>(~E(0));
+    __hip_atomic_compare_exchange_strong(p, &expected, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expected;  // the value found: all-ones when the slot was empty and now holds v
+}
+
+// Entry access used by the atomic chains: the map layout (ordinal | choice)
+// with a bucket map, or the descriptor layout (candidates | choice).
+template <int K, class M, class E>
+struct MapView {
+    const M* m;
+    __device__ __forceinline__ uint32_t bucket_of(uint64_t cur, uint32_t w) const {
+        return m[uint32_t(cur) * K + w];
+    }
+    __device__ __forceinline__ E pack(uint64_t cur, uint32_t choice) const {
+        return pack_entry<E>(uint32_t(cur), choice);
+    }
+    __device__ __forceinline__ uint64_t who(E e) const { return entry_ordinal<E>(e); }
+    __device__ __forceinline__ uint32_t choice(E e) const { return entry_choice<E>(e); }
+    __device__ __forceinline__ uint64_t item(uint32_t o) const { return o; }
+};
+template <int K>
+struct DescView {
+    DescCodec D;
+    const uint64_t* desc;
+    __device__ __forceinline__ uint32_t bucket_of(uint64_t cur, uint32_t w) const { return D.bucket(cur, w); }
+    __device__ __forceinline__ uint64_t pack(uint64_t cur, uint32_t choice) const {
+        return cur | (uint64_t(choice) << D.cs);
+    }
+    __device__ __forceinline__ uint64_t who(uint64_t e) const { return e & D.dmask; }
+    __device__ __forceinline__ uint32_t choice(uint64_t e) const { return uint32_t(e >> D.cs) & 3u; }
+    __device__ __forceinline__ uint64_t item(uint32_t o) const { return desc[o]; }
+};
+
+// BGH25 with exchange probes (same decisions as chain_bgh).
+template <int K, class E, class V>
+__device__ __forceinline__ bool chain_bgh_xchg(const Shape& S, const V& v, E* __restrict__ t, uint64_t walk,
+                                               uint64_t rnd, uint32_t& probes) {
+    uint32_t item = 0, coll = 0, stash_left = S.stash, pr = 0;
+    uint64_t cur = v.item(0);
+    uint64_t next = S.pnum > 1 ? v.item(1) : 0;
+    bool failed = false;
+    while (item < S.pnum) {
+        if (rnd < K) rnd = splitmix(walk);
+        const uint32_t which = take_choice<K>(rnd);
+        const uint32_t b = v.bucket_of(cur, which);
+        const E old = xchg(t + b, static_cast<E>(v.pack(cur, which)));
+        ++pr;
+        bool advance = old == static_cast<E>(~E(0));
+        if (!advance && ++coll > S.kick) {
+            t[b] = old;  // the reference stops before evicting: restore the occupant
+            if (!stash_left) {
+                failed = true;
+                break;
+            }
+            --stash_left;
+            advance = true;
+        }
+        if (advance) {
+            ++item;
+            cur = next;
+            if (item + 1 < S.pnum) next = v.item(item + 1);
+            coll = 0;
+        } else {
+            cur = v.who(old);
+        }
+    }
+    probes = pr;
+    return failed;
+}
+
+// DRRT18 with compare-and-swap first placements and exchange evictions.
+template <int K, class E, class V>
+__device__ __forceinline__ bool chain_drrt_xchg(const Shape& S, const V& v, E* __restrict__ t, uint32_t& probes) {
+    constexpr E kEmpty = static_cast<E>(~E(0));
+    uint32_t pr = 0, stash_left = S.stash;
+    bool failed = false;
+    uint64_t next = v.item(0);
+    for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
+        const uint64_t d = next;
+        if (item + 1 < S.pnum) next = v.item(item + 1);
+        bool placed = false;
+#pragma unroll
+        for (int w = 0; w < K; ++w) {
+            if (placed) break;
+            ++pr;
+            if (cas_empty(t + v.bucket_of(d, w), static_cast<E>(v.pack(d, w))) == kEmpty) placed = true;
+        }
+        if (placed) continue;
+        uint64_t cur = d;
+        uint32_t choice = 0, target = v.bucket_of(d, 0), kicks = 0;
+        for (;;) {
+            const E disp = xchg(t + target, static_cast<E>(v.pack(cur, choice)));
+            ++pr;
+            if (disp == kEmpty) break;
+            if (kicks >= S.kick) {
+                t[target] = disp;  // bound reached before evicting: restore the occupant
+                if (stash_left) {
+                    --stash_left;
+                    break;
+                }
+                failed = true;
+                break;
+            }
+            cur = v.who(disp);
+            choice = v.choice(disp);
+            const uint32_t prev = target;
+            bool found = false;
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+                if (found) break;
+                choice = (choice + 1 == K) ? 0 : choice + 1;
+                const uint32_t nb = v.bucket_of(cur, choice);
+                if (nb != prev) {
+                    target = nb;
+                    found = true;
+                }
+            }
+            if (!found) {
+                if (stash_left) {
+                    --stash_left;
+                    break;
+                }
+                failed = true;
+                break;
+            }
+            ++kicks;
+        }
+    }
+    probes = pr;
+    return failed;
+}
+
 // Order-independent digest of a final table: the wrapping sum over occupied
 // buckets b of mix64(b * golden ^ H ^ choice << 61), where H is the wrapping
 // sum over w < k of mix64(w << 32 | bucket(occupant, w)).  mix64 is the

@@ -128,8 +128,10 @@ __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t f
     }
 }
 
-// chain (HBM tier): one lane per trial slot; map and table widths M and E.
-template <int K, bool BGH, class M, class E>
+// chain (HBM tier): one lane per trial slot; map and table widths M and E
+// (M = u64 selects the descriptor layout).  XCHG selects one-round-trip
+// probes (atomic exchange / compare-and-swap; needs 32- or 64-bit entries).
+template <int K, bool BGH, class M, class E, bool XCHG>
 __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t count,
                                                const M* __restrict__ map, size_t map_stride,
                                                char* __restrict__ tables, size_t table_bytes,
@@ -144,10 +146,23 @@ __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t
         E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
         if constexpr (sizeof(M) == 8) {
             const DescCodec D{S.dbits, K * S.dbits, (1ull << S.dbits) - 1, (1ull << (K * S.dbits)) - 1};
-            if constexpr (BGH)
+            if constexpr (XCHG) {
+                const DescView<K> v{D, m};
+                if constexpr (BGH)
+                    failed = chain_bgh_xchg<K>(S, v, t, walks[slot], rnds[slot], pr);
+                else
+                    failed = chain_drrt_xchg<K>(S, v, t, pr);
+            } else if constexpr (BGH) {
                 failed = chain_bgh_desc<K>(S, D, m, t, walks[slot], rnds[slot], pr);
-            else
+            } else {
                 failed = chain_drrt_desc<K>(S, D, m, t, pr);
+            }
+        } else if constexpr (XCHG) {
+            const MapView<K, M, E> v{m};
+            if constexpr (BGH)
+                failed = chain_bgh_xchg<K>(S, v, t, walks[slot], rnds[slot], pr);
+            else
+                failed = chain_drrt_xchg<K>(S, v, t, pr);
         } else {
             if constexpr (BGH)
                 failed = chain_bgh<K>(S, m, t, walks[slot], rnds[slot], pr);
@@ -312,7 +327,7 @@ struct Ctx {
     uint32_t tier = CK_TIER_AUTO;
     uint32_t layout = CK_LAYOUT_AUTO;
     uint32_t chunk_cap = 0;
-    size_t budget = size_t(48) << 30;  // HBM budget for the double-buffered maps and tables
+    size_t budget = size_t(96) << 30;  // HBM budget for the double-buffered maps and tables
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
     uint32_t cus = 1;
@@ -432,17 +447,19 @@ static bool desc_ok(const ck_params& p) { return p.k * make_shape(p).dbits + 2 <
 // (double-buffered).  In the descriptor layout the "map" is one u64 candidate
 // descriptor per ordinal and the table entries are u64.
 struct HbmPlan {
-    bool desc, map16, table16;
+    bool desc, map16, table16, xchg;
     size_t map_stride;   // map entries per slot
     size_t map_bytes;    // per slot
     size_t table_bytes;  // per slot
     size_t off_cnt, off_map, off_tables, off_walk, off_rnd, off_probes, off_fail, off_key, off_digest, buffer;
 };
-static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, bool desc) {
+static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout) {
     HbmPlan h{};
+    const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     h.desc = desc;
+    h.xchg = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_MAP_XCHG;
     h.map16 = !desc && map16_ok(p);
-    h.table16 = !desc && table16_ok(p);
+    h.table16 = !desc && !h.xchg && table16_ok(p);
     const size_t mw = desc ? 8 : h.map16 ? 2 : 4, tw = desc ? 8 : h.table16 ? 2 : 4;
     h.map_bytes = align256(size_t(p.pnum) * (desc ? 1 : p.k) * mw);
     h.map_stride = h.map_bytes / mw;
@@ -473,8 +490,8 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, bool desc) {
 // Trials per pass: 65536 chains in flight saturate the random-access path
 // (measured: 32768 slower, 131072 and more no faster).
 static constexpr uint32_t kHbmChunk = 1u << 16;
-static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, bool desc) {
-    const HbmPlan one = plan_hbm(p, 1, desc);
+static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout) {
+    const HbmPlan one = plan_hbm(p, 1, layout);
     const size_t per_slot = one.map_bytes + one.table_bytes + 8 + 8 + 4 + 4 + 176;
     const size_t fit = std::max<size_t>(1, c.budget / (2 * per_slot));
     const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
@@ -555,10 +572,10 @@ static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs
         launch_build_t<K, uint32_t>(bp, a, st);
 }
 
-template <int K, bool BGH, class M, class E>
+template <int K, bool BGH, class M, class E, bool XCHG>
 static void launch_chain_t(const PassArgs& a, hipStream_t st) {
     const uint32_t blocks = (a.count + 255) / 256;
-    hipLaunchKernelGGL((k_chain<K, BGH, M, E>), dim3(blocks), dim3(256), 0, st, a.S, a.first, a.count,
+    hipLaunchKernelGGL((k_chain<K, BGH, M, E, XCHG>), dim3(blocks), dim3(256), 0, st, a.S, a.first, a.count,
                        reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds,
                        a.cnt, a.fail, a.probes);
 }
@@ -583,16 +600,22 @@ static void launch_digest_k(const HbmPlan& h, const PassArgs& a, hipStream_t st)
 
 template <int K, bool BGH>
 static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
-    if (h.desc)
-        launch_chain_t<K, BGH, uint64_t, uint64_t>(a, st);
+    if (h.desc && h.xchg)
+        launch_chain_t<K, BGH, uint64_t, uint64_t, true>(a, st);
+    else if (h.desc)
+        launch_chain_t<K, BGH, uint64_t, uint64_t, false>(a, st);
+    else if (h.xchg && h.map16)
+        launch_chain_t<K, BGH, uint16_t, uint32_t, true>(a, st);
+    else if (h.xchg)
+        launch_chain_t<K, BGH, uint32_t, uint32_t, true>(a, st);
     else if (h.map16 && h.table16)
-        launch_chain_t<K, BGH, uint16_t, uint16_t>(a, st);
+        launch_chain_t<K, BGH, uint16_t, uint16_t, false>(a, st);
     else if (h.map16)
-        launch_chain_t<K, BGH, uint16_t, uint32_t>(a, st);
+        launch_chain_t<K, BGH, uint16_t, uint32_t, false>(a, st);
     else if (h.table16)
-        launch_chain_t<K, BGH, uint32_t, uint16_t>(a, st);
+        launch_chain_t<K, BGH, uint32_t, uint16_t, false>(a, st);
     else
-        launch_chain_t<K, BGH, uint32_t, uint32_t>(a, st);
+        launch_chain_t<K, BGH, uint32_t, uint32_t, false>(a, st);
 }
 template <int K>
 static void launch_chain_k(bool bgh, const HbmPlan& h, const PassArgs& a, hipStream_t st) {
@@ -705,11 +728,13 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
 // and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
                    ck_stats* st) {
-    const bool desc = c.layout == CK_LAYOUT_DESC || (c.layout == CK_LAYOUT_AUTO && !out.tables && desc_ok(p));
+    uint32_t layout = c.layout;
+    if (layout == CK_LAYOUT_AUTO) layout = (!out.tables && desc_ok(p)) ? CK_LAYOUT_DESC : CK_LAYOUT_MAP_XCHG;
+    const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     if (desc && !desc_ok(p)) return fail(CK_EINVAL, "descriptor layout needs k*bits(buckets-1) + 2 <= 63");
     if (desc && out.tables) return fail(CK_EINVAL, "the descriptor layout does not keep ordinals; use the map layout for tables");
-    const uint32_t chunk = hbm_chunk(c, p, count, desc);
-    const HbmPlan h = plan_hbm(p, chunk, desc);
+    const uint32_t chunk = hbm_chunk(c, p, count, layout);
+    const HbmPlan h = plan_hbm(p, chunk, layout);
     int rc = ensure_ws(c, 2 * h.buffer);
     if (rc) return rc;
     const BuildPlan bp = plan_build(p, c.lds_block);
@@ -1024,7 +1049,7 @@ int ck_launch_digest(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uin
 }
 
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout) {
-    if (!ctx || layout > CK_LAYOUT_DESC) return fail(CK_EINVAL, "bad layout");
+    if (!ctx || layout > CK_LAYOUT_DESC_PLAIN) return fail(CK_EINVAL, "bad layout");
     ctx->c.layout = layout;
     return CK_OK;
 }

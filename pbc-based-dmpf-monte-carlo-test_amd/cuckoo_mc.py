@@ -18,7 +18,7 @@ CK_OK = 0
 BGH25, DRRT18 = 1, 2
 DERIVED = 0xFFFFFFFF
 TIER_AUTO, TIER_HBM, TIER_LDS = 0, 1, 2
-LAYOUT_AUTO, LAYOUT_MAP, LAYOUT_DESC = 0, 1, 2
+LAYOUT_AUTO, LAYOUT_MAP, LAYOUT_DESC, LAYOUT_MAP_XCHG, LAYOUT_DESC_PLAIN = 0, 1, 2, 3, 4
 MODES = {"bgh25": BGH25, "drrt18": DRRT18}
 
 

@@ -181,6 +181,7 @@ def test_tiers_agree_on_mixed_shapes(ctx):
 # ------------------------------------------- layouts and table digests
 
 COMBOS = {"hbm-map": (ck.TIER_HBM, ck.LAYOUT_MAP), "hbm-desc": (ck.TIER_HBM, ck.LAYOUT_DESC),
+          "hbm-map-xchg": (ck.TIER_HBM, ck.LAYOUT_MAP_XCHG), "hbm-desc-plain": (ck.TIER_HBM, ck.LAYOUT_DESC_PLAIN),
           "lds": (ck.TIER_LDS, ck.LAYOUT_AUTO)}
 
 DIGEST_SHAPES = [
