@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--trials", type=int, default=1 << 18, help="trials per step (power of two <= 2^20)")
     ap.add_argument("--tier", choices=["auto", "hbm", "lds"], default="auto")
     ap.add_argument("--chunk", type=int, default=0, help="trials per device pass (0 = library default)")
+    ap.add_argument("--layout", choices=["auto", "map", "desc", "map-xchg", "desc-plain"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -122,6 +123,8 @@ def main():
     ctx.set_tier({"auto": ck.TIER_AUTO, "hbm": ck.TIER_HBM, "lds": ck.TIER_LDS}[args.tier])
     if args.chunk:
         ctx.set_chunk(args.chunk)
+    ctx.set_layout({"auto": ck.LAYOUT_AUTO, "map": ck.LAYOUT_MAP, "desc": ck.LAYOUT_DESC,
+                    "map-xchg": ck.LAYOUT_MAP_XCHG, "desc-plain": ck.LAYOUT_DESC_PLAIN}[args.layout])
 
     if args.copy_probe_mib:
         n = args.copy_probe_mib << 18  # float32 elements
@@ -182,6 +185,7 @@ def main():
         "data": "synthetic: per-trial keys AES-derived on device from (seed 1, batch, offset)",
         "config": {"workload": "bgh25 n=16384 m=24576 k=3 stash=0 kick=500 ell_i=16 seed=1 BT=20 sequential",
                    "trials_per_step": t, "tier": {1: "hbm", 2: "lds"}.get(agg["tier"], "auto"),
+                   "layout": args.layout,
                    "trials_per_pass": agg["chunk"]},
         "insertions_per_s": value * p.pnum,
         "probes_per_s": agg["probes"] / elapsed,

@@ -487,6 +487,10 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout) {
     return h;
 }
 
+// Below this many chains per pass the exchange probes win (measured at
+// n = 2^20: 2048 chains, 595 vs 487 trials/s); above it load + store wins.
+static constexpr uint32_t kXchgBelow = 1u << 14;
+
 // Trials per pass: 65536 chains in flight saturate the random-access path
 // (measured: 32768 slower, 131072 and more no faster).
 static constexpr uint32_t kHbmChunk = 1u << 16;
@@ -728,8 +732,17 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
 // and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
                    ck_stats* st) {
+    // Auto: descriptor entries with load + store probes where they fit (fastest
+    // with tens of thousands of chains in flight); otherwise ordinal entries,
+    // with exchange probes when few chains are in flight (one round trip per
+    // probe matters when latency, not throughput, bounds the pass).
     uint32_t layout = c.layout;
-    if (layout == CK_LAYOUT_AUTO) layout = (!out.tables && desc_ok(p)) ? CK_LAYOUT_DESC : CK_LAYOUT_MAP_XCHG;
+    if (layout == CK_LAYOUT_AUTO) {
+        if (!out.tables && desc_ok(p))
+            layout = CK_LAYOUT_DESC_PLAIN;
+        else
+            layout = hbm_chunk(c, p, count, CK_LAYOUT_MAP_XCHG) < kXchgBelow ? CK_LAYOUT_MAP_XCHG : CK_LAYOUT_MAP;
+    }
     const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     if (desc && !desc_ok(p)) return fail(CK_EINVAL, "descriptor layout needs k*bits(buckets-1) + 2 <= 63");
     if (desc && out.tables) return fail(CK_EINVAL, "the descriptor layout does not keep ordinals; use the map layout for tables");
