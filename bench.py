@@ -38,6 +38,12 @@ CACHED_ROUND_OPS = 4   # one cached Feistel round (lookup index, xor, mask, swap
 EVAL_FIXED_OPS = 6     # element, y = k*e+w, cycle-walk test, division
 PROBE_OPS = 20         # choice draw (div/mod by k), table read, compare, write, advance
 SLOT_BYTES = 8         # one probe reads a 4-byte slot and writes at most one
+# HBM bytes per probe of k_chain in the descriptor layout with load + store
+# probes (the default at this shape), from FETCH_SIZE and WRITE_SIZE per
+# dispatch divided by the dispatch's probes; the chain's reads are 64-byte
+# requests, which FETCH_SIZE counts exactly (profiles/k_chain_hbm_n16384_desc_traffic.md).
+TRAFFIC_READ_PER_PROBE = 65.07
+TRAFFIC_WRITE_PER_PROBE = 27.48
 
 
 def int_ops_per_trial(p, probes_per_trial: float) -> float:
@@ -93,9 +99,10 @@ def cpu_baseline(ctx, ck, p, seconds: float):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--trials", type=int, default=1 << 18, help="trials per step (power of two <= 2^20)")
+    ap.add_argument("--trials", type=int, default=1 << BT, help="trials per step (power of two <= 2^20); "
+                    "the default is one whole batch")
     ap.add_argument("--tier", choices=["auto", "hbm", "lds"], default="auto")
     ap.add_argument("--chunk", type=int, default=0, help="trials per device pass (0 = library default)")
     ap.add_argument("--layout", choices=["auto", "map", "desc", "map-xchg", "desc-plain"], default="auto")
@@ -168,6 +175,9 @@ def main():
     chain_avg_s = agg["chain_ms"] / 1e3 / chain_launches
     bytes_per_launch = SLOT_BYTES * agg["probes"] / chain_launches
     achieved = bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else 0.0
+    measured_layout = args.layout in ("auto", "desc-plain")
+    traffic = ((TRAFFIC_READ_PER_PROBE + TRAFFIC_WRITE_PER_PROBE) * agg["probes"] / chain_launches
+               if measured_layout else None)
     probes_per_trial = agg["probes"] / trials
     ops = int_ops_per_trial(p, probes_per_trial)
     rec = {
@@ -194,8 +204,11 @@ def main():
         "roofline": {
             "kernel": "k_chain<3,bgh> (insertion chains, one lane per trial)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "avg_launch_ms": chain_avg_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "traffic_rate_gbs": traffic / chain_avg_s / 1e9 if traffic and chain_avg_s > 0 else None,
+            "note": "achieved counts 8 algorithmic bytes per probe; each probe costs one random 64-byte DRAM "
+                    "read and ~0.86 32-byte write-backs, and the chain is bound by that random-access rate",
         },
         "int_ops_model": {
             "ops_per_trial": ops, "achieved_tops": ops * value / 1e12, "peak_tops": VALU_PEAK_TOPS,
