@@ -157,6 +157,78 @@ __device__ __forceinline__ uint32_t feistel_cached(const uint16_t* cache, const 
     return div_bucket(S, y);
 }
 
+// N independent AES-128 blocks advanced round by round together (more loads in
+// flight per lane); returns the last output word of each block.
+template <int N>
+__device__ __forceinline__ void aes_last_n(const LdsTab& T, const uint32_t* rk, const uint32_t (&in)[N][4],
+                                           uint32_t (&out)[N]) {
+    uint32_t s[N][4];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s[i][j] = in[i][j] ^ rk[j];
+#pragma unroll
+    for (int r = 1; r < 10; ++r) {
+        uint32_t n[N][4];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                n[i][j] = T.t(s[i][j] >> 24) ^ rotr(T.t((s[i][(j + 1) & 3] >> 16) & 255), 8) ^
+                          rotr(T.t((s[i][(j + 2) & 3] >> 8) & 255), 16) ^ rotr(T.t(s[i][(j + 3) & 3] & 255), 24) ^
+                          rk[4 * r + j];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s[i][j] = n[i][j];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        out[i] = ((T.s(s[i][3] >> 24) << 24) | (T.s((s[i][0] >> 16) & 255) << 16) | (T.s((s[i][1] >> 8) & 255) << 8) |
+                  T.s(s[i][2] & 255)) ^
+                 rk[43];
+}
+
+// The k candidates of one element through the cached Feistel network in
+// lockstep: each pass applies the 8 rounds to every candidate (the k dependent
+// LDS chains overlap) and keeps the result only for candidates still outside
+// the domain, so each candidate ends after exactly the passes its own cycle
+// walk takes.  Same values as k calls of feistel_cached.
+template <int K>
+__device__ __forceinline__ void feistel_cached_k(const uint16_t* cache, const Shape& S, uint64_t elem,
+                                                 uint32_t (&out)[K]) {
+    const uint32_t h = S.half, m = static_cast<uint32_t>(S.mask);
+    uint32_t y[K];
+#pragma unroll
+    for (int w = 0; w < K; ++w) y[w] = static_cast<uint32_t>(elem * K + w);
+    uint32_t done = 0;
+    do {
+        uint32_t L[K], R[K];
+#pragma unroll
+        for (int w = 0; w < K; ++w) {
+            L[w] = y[w] >> h;
+            R[w] = y[w] & m;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+            for (int w = 0; w < K; ++w) {
+                const uint32_t nx = L[w] ^ cache[(r << h) + R[w]];
+                L[w] = R[w];
+                R[w] = nx & m;
+            }
+#pragma unroll
+        for (int w = 0; w < K; ++w) {
+            if (!((done >> w) & 1u)) {
+                y[w] = (L[w] << h) | R[w];
+                if (uint64_t(y[w]) < S.domain) done |= 1u << w;
+            }
+        }
+    } while (done != (1u << K) - 1u);
+#pragma unroll
+    for (int w = 0; w < K; ++w) out[w] = div_bucket(S, y[w]);
+}
+
 // Feistel PRP evaluating AES directly (any half): the round function is the
 // last word of AES_key("CESP", round, half, right) (cuckoo_bench.cu:161-184).
 __device__ __noinline__ uint32_t feistel_direct(const LdsTab& T, const uint32_t* rk, const Shape& S, uint64_t elem,

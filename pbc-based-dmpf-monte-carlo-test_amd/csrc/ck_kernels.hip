@@ -50,12 +50,32 @@ __device__ __forceinline__ void derive_uniform(const LdsTab& T, const RoundKeys&
     for (int i = 0; i < 44; ++i) ts.rk[i] = __builtin_amdgcn_readfirstlane(ts.rk[i]);
 }
 
-// The wave's round cache: cache[r<<half | right] = F_r(right) & mask.
+// The wave's round cache: cache[r<<half | right] = F_r(right) & mask.  Each
+// lane encrypts four entries at a time (entries is a multiple of 256 for
+// half >= 5), one per quarter of the cache.
 __device__ __forceinline__ void build_cache(const LdsTab& T, const TrialSeed& ts, const Shape& S, uint16_t* cache,
                                             uint32_t lane) {
     const uint32_t entries = 8u << S.half, rmask = (1u << S.half) - 1u;
-    for (uint32_t e = lane; e < entries; e += 64)
-        cache[e] = static_cast<uint16_t>(aes_last(T, ts.rk, 0x43455350u, e >> S.half, S.half, e & rmask) & S.mask);
+    if ((entries & 255u) == 0) {
+        const uint32_t q = entries >> 2;
+        for (uint32_t e = lane; e < q; e += 64) {
+            uint32_t in[4][4], out[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t x = e + i * q;
+                in[i][0] = 0x43455350u;
+                in[i][1] = x >> S.half;
+                in[i][2] = S.half;
+                in[i][3] = x & rmask;
+            }
+            aes_last_n<4>(T, ts.rk, in, out);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cache[e + i * q] = static_cast<uint16_t>(out[i] & S.mask);
+        }
+    } else {
+        for (uint32_t e = lane; e < entries; e += 64)
+            cache[e] = static_cast<uint16_t>(aes_last(T, ts.rk, 0x43455350u, e >> S.half, S.half, e & rmask) & S.mask);
+    }
     wave_lds_sync();
 }
 
@@ -79,7 +99,7 @@ __device__ __forceinline__ void store_candidates(M* m, const Shape& S, uint32_t 
 // descriptor layout (one packed candidate list per ordinal); the slot's table
 // (table_bytes, a multiple of 16) is filled with all-ones (empty).
 template <int K, bool CACHED, class M>
-__global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
+__global__ __launch_bounds__(1024) void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
                                                uint32_t tab_shift, M* __restrict__ map, size_t map_stride,
                                                char* __restrict__ tables, size_t table_bytes,
                                                uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
@@ -104,10 +124,8 @@ __global__ __launch_bounds__(256) void k_build(Shape S, RoundKeys bk, uint32_t f
         uint16_t* cache = reinterpret_cast<uint16_t*>(lds + (256u << tab_shift)) + (size_t(wave) << (S.half + 3));
         build_cache(T, ts, S, cache, lane);
         for (uint32_t o = lane; o < S.pnum; o += 64) {
-            const uint64_t el = element_of(S, ts.shift, ts.step, o);
             uint32_t c[K];
-#pragma unroll
-            for (int w = 0; w < K; ++w) c[w] = feistel_cached(cache, S, el, K, w);
+            feistel_cached_k<K>(cache, S, element_of(S, ts.shift, ts.step, o), c);
             store_candidates<K>(m, S, o, c);
         }
     } else {
@@ -236,9 +254,10 @@ __global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t fir
         derive_uniform(T, bk, S, first + slot, ts);
         build_cache(T, ts, S, cache, lane);
         for (uint32_t o = lane; o < S.pnum; o += 64) {
-            const uint64_t el = element_of(S, ts.shift, ts.step, o);
+            uint32_t c[K];
+            feistel_cached_k<K>(cache, S, element_of(S, ts.shift, ts.step, o), c);
 #pragma unroll
-            for (int w = 0; w < K; ++w) map[o * K + w] = static_cast<M>(feistel_cached(cache, S, el, K, w));
+            for (int w = 0; w < K; ++w) map[o * K + w] = static_cast<M>(c[w]);
         }
         for (uint32_t i = lane; i < S.buckets; i += 64) table[i] = empty_entry<E>();
         wave_lds_sync();
@@ -375,6 +394,10 @@ static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 static bool map16_ok(const ck_params& p) { return p.buckets <= 65536; }
 static bool table16_ok(const ck_params& p) { return p.pnum < 16384 || (p.pnum == 16384 && p.k <= 3); }
 
+// Trials (waves) per build block: up to 16, as LDS allows next to the T-table,
+// so a CU keeps enough waves to hide the LDS latency of the AES rounds.
+static constexpr uint32_t kBuildWaves = 16;
+
 struct BuildPlan {
     bool cached;
     uint32_t waves;  // trials per build block
@@ -386,7 +409,7 @@ static BuildPlan plan_build(const ck_params& p, size_t lds_block) {
     const size_t cache = (size_t(8) << p.half_bits) * sizeof(uint16_t);
     b.cached = p.half_bits <= kMaxCacheHalf && tab + cache <= lds_block;
     if (b.cached) {
-        b.waves = static_cast<uint32_t>(std::min<size_t>(4, (lds_block - tab) / cache));
+        b.waves = static_cast<uint32_t>(std::min<size_t>(kBuildWaves, (lds_block - tab) / cache));
         b.lds = tab + b.waves * cache;
     } else {
         b.waves = 4;

@@ -1,0 +1,116 @@
+"""End-to-end runs of the cuckoo_bench command line on the GPU (the
+counterpart of the reference's smoke_test.py): golden failure sets through the
+journals, several workers, a mid-run PAUSE and resume with other workers and
+micro size, a replay of a complete directory, and journals the reference's own
+cpu_bench accepts (oracle/_ref, when present)."""
+import json
+import subprocess
+import time
+from pathlib import Path
+
+import pytest
+
+from paths import PKG, ROOT
+
+pytestmark = pytest.mark.gpu
+
+CLI = PKG / "lib" / "cuckoo_bench"
+REF_CPU_BENCH = ROOT / "oracle" / "_ref" / "cpu_bench"
+SMOKE = ["--et", "13", "--bt", "8", "--pnum", "16", "--ell-i", "8", "--seed", "123"]
+
+
+def run(args, timeout=120):
+    out = subprocess.run([str(CLI), *args], capture_output=True, text=True, timeout=timeout)
+    assert out.returncode == 0, out.stderr[-2000:]
+    return out
+
+
+def failures(d: Path):
+    lines = (d / "failures.csv").read_text().splitlines()
+    assert lines[0] == "batch_id,offset"
+    rows = [tuple(map(int, x.split(","))) for x in lines[1:]]
+    assert len(rows) == len(set(rows))
+    return sorted(b * 256 + o for b, o in rows)
+
+
+def summary(d: Path):
+    return json.loads((d / "summary.json").read_text())
+
+
+@pytest.mark.parametrize("mode", ["bgh25", "drrt18"])
+def test_cli_golden_and_workers(tmp_path, fixtures, mode):
+    want = fixtures["smoke_expected"]["failures"][mode]
+    for name, extra in [("baseline", ["--workers", "1", "--micro", "257"]),
+                        ("multi", ["--workers", "2", "--micro", "37", "--tier", "hbm"]),
+                        ("lds", ["--workers", "1", "--micro", "100", "--tier", "lds"])]:
+        d = tmp_path / name
+        out = run(["--mode", mode, *SMOKE, *extra, "--stop-after", "8192", "--out", str(d)])
+        assert "RESULT status=complete" in out.stdout
+        s = summary(d)
+        assert s["status"] == "complete" and s["completed"] == 8192 and s["completed_batches"] == 32
+        assert s["failed"] == len(want)
+        assert failures(d) == want, name
+
+
+@pytest.mark.parametrize("mode", ["bgh25", "drrt18"])
+def test_cli_pause_and_resume(tmp_path, fixtures, mode):
+    d = tmp_path / "resumed"
+    d.mkdir()
+    cmd = [str(CLI), "--mode", mode, *SMOKE, "--workers", "1", "--micro", "1", "--stop-after", "8192",
+           "--out", str(d)]
+    proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    try:
+        deadline = time.monotonic() + 120
+        while proc.poll() is None:
+            # a 56-byte v3 header plus one complete empty record (24 bytes)
+            if any(p.stat().st_size >= 80 for p in d.glob("worker_*.bin")):
+                (d / "PAUSE").touch()
+                break
+            assert time.monotonic() < deadline
+            time.sleep(0.005)
+        proc.wait(timeout=120)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+            proc.wait()
+    assert proc.returncode == 0
+    s = summary(d)
+    if s["status"] == "paused":
+        assert 0 < s["completed"] < 8192 and s["completed"] % 256 == 0
+    (d / "PAUSE").unlink(missing_ok=True)
+    run(["--mode", mode, *SMOKE, "--workers", "2", "--micro", "113", "--stop-after", "8192", "--out", str(d)])
+    s = summary(d)
+    assert s["status"] == "complete" and s["completed"] == 8192
+    want = fixtures["smoke_expected"]["failures"][mode]
+    assert failures(d) == want
+    # Replaying a complete directory performs no work and changes nothing.
+    before = {p.name: p.stat().st_size for p in d.glob("worker_*.bin")}
+    run(["--mode", mode, *SMOKE, "--workers", "1", "--micro", "19", "--stop-after", "8192", "--out", str(d)])
+    assert {p.name: p.stat().st_size for p in d.glob("worker_*.bin")} == before
+    assert failures(d) == want
+    if REF_CPU_BENCH.exists():
+        ref = subprocess.run([str(REF_CPU_BENCH), "--mode", mode, *SMOKE, "--status", "--out", str(d)],
+                             capture_output=True, text=True, timeout=60)
+        assert ref.returncode == 0, ref.stderr
+        assert json.loads((d / "summary.json").read_text())["completed"] == 8192
+
+
+def test_cli_extension_journal(tmp_path):
+    # Non-power-of-two plan, k = 4, stash 1: v4 journal, resume to completion.
+    d = tmp_path / "ext"
+    args = ["--mode", "bgh25", "--pnum", "200", "--ell-i", "12", "--k", "4", "--buckets", "230", "--kick", "5",
+            "--stash", "1", "--trials", "1000", "--bt", "8", "--seed", "9", "--out", str(d)]
+    run(args)
+    s = summary(d)
+    assert s["status"] == "complete" and s["completed"] == 1000 and s["journal_version"] == 4
+    assert s["K"] == 4 and s["STASH"] == 1 and s["planned"] == 1000
+    import sys
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ck_oracle as ock
+    op = ock.parameters("bgh25", 200, 12, k=4, buckets=230, kick=5, stash=1)
+    want = []
+    for b in range(4):
+        n = min(256, 1000 - b * 256)
+        f, _, _ = ock.run_range(ock.batch_key(9, b), 0, n, op)
+        want += [b * 256 + int(x) for x in f]
+    assert failures(d) == want
