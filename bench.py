@@ -96,6 +96,107 @@ def cpu_baseline(ctx, ck, p, seconds: float):
     }, parity
 
 
+def load_sweep():
+    import importlib.util
+    from paths import PKG
+    spec = importlib.util.spec_from_file_location("ck_sweep", PKG / "sweep.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    i = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def sweep_leg(ck, points: int, trials: int):
+    """Mixed sweep (seed 2): one engine run over all points against one
+    single-shape launch per point, same trials; failure counts must agree."""
+    sw = load_sweep()
+    pts = sw.sweep_points(points, seed=2, trials=trials)
+    ps = [sw.params_of(ck, pt) for pt in pts]
+    ctx = ck.Context(0)
+    t0 = time.perf_counter()
+    plain = []
+    for pt, p in zip(pts, ps):
+        f, st = ctx.launch(p, ck.batch_key(pt["seed"], 0), 0, pt["trials"])
+        plain.append((len(f), st.probes))
+    plain_s = time.perf_counter() - t0
+    ctx.close()
+    print(f"sweep: per-point launches {plain_s:.2f} s", file=sys.stderr, flush=True)
+    eng = ck.Engine(0, policy=ck.SCHED_BALANCE)
+    t0 = time.perf_counter()
+    ids = [eng.submit(p, pt["seed"], pt["trials"], bt=pt["bt"]) for pt, p in zip(pts, ps)]
+    res = [eng.wait(i) for i in ids]
+    eng_s = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    print(f"sweep: engine {eng_s:.2f} s", file=sys.stderr, flush=True)
+    equal = all((r.failed, r.probes) == pl for r, pl in zip(res, plain))
+    total = sum(pt["trials"] for pt in pts)
+    return {
+        "workload": f"{points} points from seed 2 (n log-uniform [2^8,2^16], k 2..4, m/n U[1.2,1.6], stash 0..4, "
+                    f"kick 100/500/1000, bgh25, ell_i 16), {trials} trials each",
+        "jobs_per_s": points / eng_s, "trials_per_s": total / eng_s,
+        "plain_jobs_per_s": points / plain_s, "plain_trials_per_s": total / plain_s,
+        "speedup_vs_plain": plain_s / eng_s,
+        "engine_s": eng_s, "plain_s": plain_s, "passes": st.passes,
+        "device_build_ms": st.build_ms, "device_chain_ms": st.chain_ms,
+        "failed": sum(r.failed for r in res), "results_equal_plain": equal,
+    }, equal
+
+
+def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float):
+    """Poisson arrivals (seed 3) of sweep-shaped jobs: the engine (continuous
+    batching, least-remaining-work first) against a plain loop that launches
+    one job at a time in arrival order.  Latency = completion - arrival."""
+    sw = load_sweep()
+    shapes = sw.sweep_points(jobs, seed=3)
+    ps = [sw.params_of(ck, pt) for pt in shapes]
+    out = []
+    ok = True
+    for rate in rates:
+        arr = sw.arrivals(jobs, rate, seed=3, budget_lo=budget_lo, budget_hi=budget_hi)
+        eng = ck.Engine(0, policy=ck.SCHED_SRPT)
+        t0 = time.perf_counter()
+        ids = []
+        for (at, budget), pt, p in zip(arr, shapes, ps):
+            d = at - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids.append(eng.submit(p, pt["seed"], budget))
+        res = [eng.wait(i) for i in ids]
+        eng.close()
+        lat = [r.end_ms - r.submit_ms for r in res]
+        span = (max(r.end_ms for r in res) - min(r.submit_ms for r in res)) / 1e3
+        # Plain loop: one launch at a time, in arrival order.
+        ctx = ck.Context(0)
+        t0 = time.perf_counter()
+        plat, pres = [], []
+        for (at, budget), pt, p in zip(arr, shapes, ps):
+            d = at - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            f, st = ctx.launch(p, ck.batch_key(pt["seed"], 0), 0, budget)
+            pres.append((len(f), st.probes))
+            plat.append((time.perf_counter() - t0 - at) * 1e3)
+        pspan = time.perf_counter() - t0
+        ctx.close()
+        equal = all((r.failed, r.probes) == q for r, q in zip(res, pres))
+        ok = ok and equal
+        out.append({"rate": rate, "jobs": jobs, "jobs_per_s": jobs / span, "p50_ms": pct(lat, 50),
+                    "p99_ms": pct(lat, 99), "plain_jobs_per_s": jobs / pspan, "plain_p50_ms": pct(plat, 50),
+                    "plain_p99_ms": pct(plat, 99), "results_equal_plain": equal})
+        print(f"serve: rate {rate}: p50 {pct(lat, 50):.1f} ms p99 {pct(lat, 99):.1f} ms "
+              f"(plain {pct(plat, 50):.1f} / {pct(plat, 99):.1f})", file=sys.stderr, flush=True)
+    return {"workload": f"Poisson arrivals (seed 3), shapes as the sweep (seed 3), trial budgets log-uniform "
+                        f"[{budget_lo:g}, {budget_hi:g}]", "rates": out}, ok
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +209,14 @@ def main():
     ap.add_argument("--layout", choices=["auto", "map", "desc", "map-xchg", "desc-plain"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-points", type=int, default=512)
+    ap.add_argument("--sweep-trials", type=int, default=1 << 15,
+                    help="trials per sweep point (the full sweep is 10^6)")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--serve-rates", default="10,100,1000")
+    ap.add_argument("--serve-jobs", type=int, default=100)
+    ap.add_argument("--serve-budget", default="10,1e5", help="trial budget range lo,hi (log-uniform)")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before warmup, copy a buffer of this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -220,6 +329,13 @@ def main():
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)
     ctx.close()
+    if not args.no_sweep:
+        rec["sweep"], good = sweep_leg(ck, args.sweep_points, args.sweep_trials)
+        ok = ok and good
+    if not args.no_serve:
+        lo, hi = (float(x) for x in args.serve_budget.split(","))
+        rec["serving"], good = serve_leg(ck, [float(r) for r in args.serve_rates.split(",")], args.serve_jobs, lo, hi)
+        ok = ok and good
     print(json.dumps(rec), flush=True)
     return 0 if ok else 1
 

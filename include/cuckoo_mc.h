@@ -134,6 +134,81 @@ int ck_trial_map(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44],
 
 const char* ck_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * Batched and served sweeps.  The reference runs one experiment shape per
+ * process (cuckoo_bench.cu:1062-1142, driven per point by tune_bench.py and
+ * compare_cpu_gpu.py); the engine below runs many shapes ("jobs") through
+ * shared device passes.  Each job is the same trial sequence as the launch
+ * callable: trial t is offset t mod 2^bt of batch t >> bt, keyed by
+ * batch_key(seed, batch) (cuckoo_bench.cu:393-404, :728-734), so a job's
+ * failing trials are exactly those ck_launch reports for the same ranges.
+ * ---------------------------------------------------------------------- */
+
+enum { CK_PENDING = 1 }; /* ck_engine_poll: the job has not finished */
+
+enum {
+    CK_SCHED_BALANCE = 0, /* overlap build-heavy and chain-heavy passes (throughput) */
+    CK_SCHED_FIFO = 1,    /* oldest job first */
+    CK_SCHED_SRPT = 2     /* least remaining estimated work first (latency) */
+};
+
+typedef struct ck_job {
+    ck_params p;            /* shape, as returned by ck_params_derive */
+    uint64_t seed;          /* master seed of the batch keys */
+    uint64_t trials;        /* trials 0 .. trials-1 */
+    uint32_t bt;            /* batch bits, 6..32 */
+    uint32_t keep_failures; /* nonzero: keep the failing trial indices (ck_engine_failures) */
+} ck_job;
+
+typedef struct ck_job_result {
+    uint64_t trials;   /* trials run (device counter) */
+    uint64_t failed;   /* failing trials (device counter) */
+    uint64_t probes;   /* table-slot reads of the insertion chains (device counter) */
+    double submit_ms;  /* engine clock at submission */
+    double start_ms;   /* first pass holding the job enqueued */
+    double end_ms;     /* last pass gathered; counters final */
+} ck_job_result;
+
+typedef struct ck_engine_config {
+    uint64_t arena_bytes; /* device memory for maps and tables over all pass buffers (0: 96 GiB) */
+    uint32_t buffers;     /* pass buffers in flight, 2..8 (0: 3) */
+    uint32_t max_slots;   /* trial slots per pass, multiple of 64 (0: 65536) */
+    uint32_t max_jobs;    /* jobs admitted at once: device counter slots (0: 4096) */
+    uint32_t policy;      /* CK_SCHED_* */
+    uint32_t window;      /* jobs considered per pass, in arrival order (0: all) */
+    uint32_t pad;
+} ck_engine_config;
+
+typedef struct ck_engine_stats {
+    uint64_t passes, segments, slots;       /* passes run, 64-trial segments, live trial slots */
+    uint64_t jobs_submitted, jobs_done;
+    uint64_t trials, failed, probes;        /* totals over finished jobs */
+    double build_ms, chain_ms;              /* device time summed over passes */
+    double wall_ms;                         /* engine clock now */
+} ck_engine_stats;
+
+typedef struct ck_engine ck_engine;
+
+/* An engine owns a device context, its pass buffers (ring of `buffers`),
+ * device-resident per-job counters and a scheduler thread. */
+int ck_engine_create(int device, const ck_engine_config* cfg, ck_engine** out);
+/* Finishes the passes in flight, then stops; unfinished jobs are abandoned. */
+void ck_engine_destroy(ck_engine* e);
+/* Queues a job (thread-safe).  Jobs are admitted as device counter slots
+ * free up and join the next pass planned.  CK_ENOMEM: 64 trials of the shape
+ * do not fit one pass buffer. */
+int ck_engine_submit(ck_engine* e, const ck_job* job, uint64_t* job_id);
+/* CK_OK and the result when the job has finished, CK_PENDING otherwise. */
+int ck_engine_poll(ck_engine* e, uint64_t job_id, ck_job_result* out);
+/* Blocks until the job has finished. */
+int ck_engine_wait(ck_engine* e, uint64_t job_id, ck_job_result* out);
+/* Failing trial indices of a finished job kept with keep_failures, ascending:
+ * out[0 .. min(cap, n)-1], *n = their number. */
+int ck_engine_failures(ck_engine* e, uint64_t job_id, uint64_t* out, uint64_t cap, uint64_t* n);
+/* Drops a finished job's host record. */
+int ck_engine_release(ck_engine* e, uint64_t job_id);
+int ck_engine_stats_get(ck_engine* e, ck_engine_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,92 @@
+// ck_build.hpp -- the per-trial build steps shared by the single-shape kernels
+// and the mixed-job engine: wave-uniform trial derivation, the wave's round
+// cache, and the bucket-map row store.
+#pragma once
+#include "ck_device.hpp"
+#include "cuckoo_mc.h"
+
+namespace ck {
+
+// Kernel-argument shape from validated params (host).
+inline Shape make_shape(const ck_params& p) {
+    Shape s{};
+    s.k = p.k;
+    s.pnum = p.pnum;
+    s.buckets = p.buckets;
+    s.kick = p.kick;
+    s.stash = p.stash;
+    s.half = p.half_bits;
+    s.input_mode = p.input_mode;
+    s.mode = p.mode;
+    s.bucket_size = p.bucket_size;
+    s.domain = p.domain;
+    s.mask = p.mask;
+    s.range_mask = p.range_mask;
+    s.div_magic = p.bucket_size <= 1 ? 0 : (~0ull / p.bucket_size) + 1;
+    uint32_t bb = 1;
+    while (bb < 32 && (uint64_t(1) << bb) < p.buckets) ++bb;
+    s.dbits = bb;
+    return s;
+}
+
+
+// Stages T0 (2^shift copies) from `src` (constant or global memory) into LDS;
+// returns the lane's view.
+__device__ __forceinline__ LdsTab stage_tab_from(const uint32_t* src, uint32_t* lds, uint32_t shift) {
+    const uint32_t n = 256u << shift;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = src[i >> shift];
+    __syncthreads();
+    return LdsTab{lds, threadIdx.x & ((1u << shift) - 1u), shift};
+}
+
+__device__ __forceinline__ void derive_uniform(const LdsTab& T, const RoundKeys& bk, const Shape& S, uint32_t offset,
+                                               TrialSeed& ts) {
+    derive_trial(T, bk, S, offset, ts);
+#pragma unroll
+    for (int i = 0; i < 44; ++i) ts.rk[i] = __builtin_amdgcn_readfirstlane(ts.rk[i]);
+}
+
+// The wave's round cache: cache[r<<half | right] = F_r(right) & mask.  Each
+// lane encrypts four entries at a time (entries is a multiple of 256 for
+// half >= 5), one per quarter of the cache.
+__device__ __forceinline__ void build_cache(const LdsTab& T, const TrialSeed& ts, const Shape& S, uint16_t* cache,
+                                            uint32_t lane) {
+    const uint32_t entries = 8u << S.half, rmask = (1u << S.half) - 1u;
+    if ((entries & 255u) == 0) {
+        const uint32_t q = entries >> 2;
+        for (uint32_t e = lane; e < q; e += 64) {
+            uint32_t in[4][4], out[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t x = e + i * q;
+                in[i][0] = 0x43455350u;
+                in[i][1] = x >> S.half;
+                in[i][2] = S.half;
+                in[i][3] = x & rmask;
+            }
+            aes_last_n<4>(T, ts.rk, in, out);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cache[e + i * q] = static_cast<uint16_t>(out[i] & S.mask);
+        }
+    } else {
+        for (uint32_t e = lane; e < entries; e += 64)
+            cache[e] = static_cast<uint16_t>(aes_last(T, ts.rk, 0x43455350u, e >> S.half, S.half, e & rmask) & S.mask);
+    }
+    wave_lds_sync();
+}
+
+// The bucket map row of ordinal o: k entries, or one packed descriptor.
+template <int K, class M>
+__device__ __forceinline__ void store_candidates(M* m, const Shape& S, uint32_t o, const uint32_t (&c)[K]) {
+    if constexpr (sizeof(M) == 8) {
+        uint64_t d = 0;
+#pragma unroll
+        for (int w = 0; w < K; ++w) d |= uint64_t(c[w]) << (w * S.dbits);
+        m[o] = d;
+    } else {
+#pragma unroll
+        for (int w = 0; w < K; ++w) m[o * K + w] = static_cast<M>(c[w]);
+    }
+}
+
+}  // namespace ck

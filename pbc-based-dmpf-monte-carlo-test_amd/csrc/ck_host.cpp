@@ -190,6 +190,15 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+int check_params(const ck_params* p) {
+    if (!p) return fail(CK_EINVAL, "null params");
+    ck_params x;
+    const int rc = derive_params(p->mode, p->pnum, p->ell_i, p->k, p->buckets, p->kick, p->stash, p->input_mode, &x);
+    if (rc) return rc;
+    if (std::memcmp(&x, p, sizeof x) != 0) return fail(CK_EINVAL, "params are not the output of ck_params_derive");
+    return CK_OK;
+}
+
 }  // namespace ck
 
 extern "C" {

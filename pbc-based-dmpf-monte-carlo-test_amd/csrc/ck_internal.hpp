@@ -20,6 +20,8 @@ int derive_params(uint32_t mode, uint32_t pnum, uint32_t ell_i, uint32_t k, uint
                   uint32_t stash, uint32_t input_mode, ck_params* out);
 
 int fail(int code, const std::string& msg);
+// CK_OK when *p is exactly what ck_params_derive returns for its inputs.
+int check_params(const ck_params* p);
 void set_error(const std::string& e);
 const char* last_error();
 

@@ -18,6 +18,8 @@ CK_OK = 0
 BGH25, DRRT18 = 1, 2
 DERIVED = 0xFFFFFFFF
 TIER_AUTO, TIER_HBM, TIER_LDS = 0, 1, 2
+PENDING = 1
+SCHED_BALANCE, SCHED_FIFO, SCHED_SRPT = 0, 1, 2
 LAYOUT_AUTO, LAYOUT_MAP, LAYOUT_DESC, LAYOUT_MAP_XCHG, LAYOUT_DESC_PLAIN = 0, 1, 2, 3, 4
 MODES = {"bgh25": BGH25, "drrt18": DRRT18}
 
@@ -36,6 +38,33 @@ class Stats(C.Structure):
                 ("tier", C.c_uint32), ("chunk", C.c_uint32), ("build_ms", C.c_double),
                 ("chain_ms", C.c_double), ("fused_ms", C.c_double), ("launches", C.c_uint32),
                 ("pad", C.c_uint32)]
+
+
+class Job(C.Structure):
+    _fields_ = [("p", Params), ("seed", C.c_uint64), ("trials", C.c_uint64), ("bt", C.c_uint32),
+                ("keep_failures", C.c_uint32)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("trials", C.c_uint64), ("failed", C.c_uint64), ("probes", C.c_uint64),
+                ("submit_ms", C.c_double), ("start_ms", C.c_double), ("end_ms", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [("arena_bytes", C.c_uint64), ("buffers", C.c_uint32), ("max_slots", C.c_uint32),
+                ("max_jobs", C.c_uint32), ("policy", C.c_uint32), ("window", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class EngineStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("passes", "segments", "slots", "jobs_submitted", "jobs_done", "trials",
+                                          "failed", "probes")] + \
+               [(n, C.c_double) for n in ("build_ms", "chain_ms", "wall_ms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class CkError(RuntimeError):
@@ -67,6 +96,14 @@ def _load():
                                       P(Stats)]),
         "ck_trial_map": (C.c_int, [vp, P(Params), P(U32), U32, P(U32), P(U64), P(U64), P(U32)]),
         "ck_last_error": (C.c_char_p, []),
+        "ck_engine_create": (C.c_int, [C.c_int, P(EngineConfig), P(vp)]),
+        "ck_engine_destroy": (None, [vp]),
+        "ck_engine_submit": (C.c_int, [vp, P(Job), P(U64)]),
+        "ck_engine_poll": (C.c_int, [vp, U64, P(JobResult)]),
+        "ck_engine_wait": (C.c_int, [vp, U64, P(JobResult)]),
+        "ck_engine_failures": (C.c_int, [vp, U64, P(U64), U64, P(U64)]),
+        "ck_engine_release": (C.c_int, [vp, U64]),
+        "ck_engine_stats_get": (C.c_int, [vp, P(EngineStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -80,7 +117,9 @@ LIB = _load()
 # Every symbol declared in include/cuckoo_mc.h.
 EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "ck_ctx_destroy",
            "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_ctx_set_layout", "ck_aes_self_test", "ck_launch",
-           "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error"]
+           "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_engine_create",
+           "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
+           "ck_engine_release", "ck_engine_stats_get"]
 
 
 def _check(rc):
@@ -190,3 +229,68 @@ class Context:
         _check(LIB.ck_trial_map(self.h, C.byref(p), _u32p(rk), offset, _u32p(key), C.byref(walk),
                                 C.byref(rnd), _u32p(m)))
         return key, walk.value, rnd.value, m
+
+
+class Engine:
+    """Batched / served sweeps: jobs of any shape share device passes; results
+    are the same trial-for-trial as Context.launch over the same ranges."""
+
+    def __init__(self, device: int = 0, arena_bytes: int = 0, buffers: int = 0, max_slots: int = 0,
+                 max_jobs: int = 0, policy: int = SCHED_BALANCE, window: int = 0):
+        cfg = EngineConfig(arena_bytes, buffers, max_slots, max_jobs, policy, window, 0)
+        h = C.c_void_p()
+        _check(LIB.ck_engine_create(device, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            LIB.ck_engine_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, p: Params, seed: int, trials: int, bt: int = 20, keep_failures: bool = False) -> int:
+        job = Job(p, seed, trials, bt, 1 if keep_failures else 0)
+        jid = C.c_uint64()
+        _check(LIB.ck_engine_submit(self.h, C.byref(job), C.byref(jid)))
+        return jid.value
+
+    def poll(self, job_id: int):
+        """The JobResult when the job has finished, else None."""
+        r = JobResult()
+        rc = LIB.ck_engine_poll(self.h, job_id, C.byref(r))
+        if rc == PENDING:
+            return None
+        _check(rc)
+        return r
+
+    def wait(self, job_id: int) -> JobResult:
+        r = JobResult()
+        _check(LIB.ck_engine_wait(self.h, job_id, C.byref(r)))
+        return r
+
+    def failures(self, job_id: int) -> np.ndarray:
+        n = C.c_uint64()
+        _check(LIB.ck_engine_failures(self.h, job_id, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(LIB.ck_engine_failures(self.h, job_id, out.ctypes.data_as(C.POINTER(C.c_uint64)), n.value,
+                                      C.byref(n)))
+        return out[:n.value].copy()
+
+    def release(self, job_id: int):
+        _check(LIB.ck_engine_release(self.h, job_id))
+
+    def stats(self) -> EngineStats:
+        s = EngineStats()
+        _check(LIB.ck_engine_stats_get(self.h, C.byref(s)))
+        return s

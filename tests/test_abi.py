@@ -72,3 +72,30 @@ def test_ctx_create_without_device_fails_cleanly():
     h = C.c_void_p()
     rc = ck.LIB.ck_ctx_create(0, C.byref(h))
     assert rc != 0 and not h.value
+
+
+def test_engine_create_without_device_fails_cleanly():
+    import paths
+    if paths.gpu_available():
+        pytest.skip("a device is present")
+    h = C.c_void_p()
+    rc = ck.LIB.ck_engine_create(0, None, C.byref(h))
+    assert rc != 0 and not h.value
+    assert ck.LIB.ck_last_error()
+
+
+def test_engine_config_validation():
+    h = C.c_void_p()
+    for bad in (dict(buffers=1), dict(buffers=9), dict(max_slots=100), dict(policy=3)):
+        cfg = ck.EngineConfig(**bad)
+        assert ck.LIB.ck_engine_create(0, C.byref(cfg), C.byref(h)) == -1, bad
+        assert not h.value
+
+
+def test_struct_layouts():
+    # ck_job = ck_params (72 B) + seed + trials + bt + keep_failures
+    assert C.sizeof(ck.Params) == 72
+    assert C.sizeof(ck.Job) == 96
+    assert C.sizeof(ck.JobResult) == 48
+    assert C.sizeof(ck.EngineConfig) == 32
+    assert C.sizeof(ck.EngineStats) == 88
