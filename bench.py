@@ -113,6 +113,13 @@ def pct(xs, q):
     return xs[i]
 
 
+def prewarm(ck, ctx, pts, ps):
+    """Grows the context's workspace to the largest shape once, so that the
+    timed per-point launches do not include device allocation."""
+    big = max(range(len(pts)), key=lambda i: (ps[i].pnum + ps[i].buckets))
+    ctx.launch(ps[big], ck.batch_key(pts[big]["seed"], 0), 0, min(pts[big]["trials"], 1 << 16))
+
+
 def sweep_leg(ck, points: int, trials: int):
     """Mixed sweep (seed 2): one engine run over all points against one
     single-shape launch per point, same trials; failure counts must agree."""
@@ -120,6 +127,7 @@ def sweep_leg(ck, points: int, trials: int):
     pts = sw.sweep_points(points, seed=2, trials=trials)
     ps = [sw.params_of(ck, pt) for pt in pts]
     ctx = ck.Context(0)
+    prewarm(ck, ctx, pts, ps)
     t0 = time.perf_counter()
     plain = []
     for pt, p in zip(pts, ps):
@@ -146,7 +154,8 @@ def sweep_leg(ck, points: int, trials: int):
         "speedup_vs_plain": plain_s / eng_s,
         "engine_s": eng_s, "plain_s": plain_s, "passes": st.passes,
         "device_build_ms": st.build_ms, "device_chain_ms": st.chain_ms,
-        "failed": sum(r.failed for r in res), "results_equal_plain": equal,
+        "failed": sum(r.failed for r in res), "probes": sum(r.probes for r in res),
+        "probes_per_s": sum(r.probes for r in res) / eng_s, "results_equal_plain": equal,
     }, equal
 
 
@@ -175,6 +184,7 @@ def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float):
         span = (max(r.end_ms for r in res) - min(r.submit_ms for r in res)) / 1e3
         # Plain loop: one launch at a time, in arrival order.
         ctx = ck.Context(0)
+        prewarm(ck, ctx, [dict(pt, trials=b) for pt, (_, b) in zip(shapes, arr)], ps)
         t0 = time.perf_counter()
         plat, pres = [], []
         for (at, budget), pt, p in zip(arr, shapes, ps):

@@ -170,8 +170,9 @@ typedef struct ck_job_result {
 } ck_job_result;
 
 typedef struct ck_engine_config {
-    uint64_t arena_bytes; /* device memory for maps and tables over all pass buffers (0: 96 GiB) */
-    uint32_t buffers;     /* pass buffers in flight, 2..8 (0: 3) */
+    uint64_t arena_bytes; /* device memory for maps and tables over all pass buffers
+                             (0: 192 GiB or five sixths of the free memory, the smaller) */
+    uint32_t buffers;     /* pass buffers in flight, 2..8 (0: 4); passes chain on two streams */
     uint32_t max_slots;   /* trial slots per pass, multiple of 64 (0: 65536) */
     uint32_t max_jobs;    /* jobs admitted at once: device counter slots (0: 4096) */
     uint32_t policy;      /* CK_SCHED_* */

@@ -8,12 +8,14 @@
 // share the kernel instance: k, insertion policy, table layout and the
 // cached/direct PRP form (the job's "group").  Each pass has its own arena for
 // the maps and tables of its slots, its walk/randomness arrays and its failure
-// list; a ring of pass buffers lets the build of later passes run on one
-// stream while earlier passes chain on the other.
+// list.  A ring of pass buffers lets the builds of later passes run on one
+// stream while earlier passes chain on two others: consecutive passes chain on
+// alternate streams, so the slowest trials of one pass (its tail) overlap the
+// bulk of the next and the random-access path stays saturated.
 //
 // Per-job counters (failures, probes, trials) live in device memory: the
-// chain waves add their ballot counts to their job's slot, and the host reads
-// a mirror copied behind each pass.  A scheduler thread admits queued jobs as
+// chain waves add their ballot counts to their job's slot; the host reads them
+// once every pass holding the job has finished.  A scheduler thread admits queued jobs as
 // counter slots free up, plans passes into free buffers (continuous batching:
 // jobs that arrive while others run join the next pass), and gathers finished
 // passes in order.
@@ -204,7 +206,7 @@ struct PassBuf {
     uint64_t* rnds = nullptr;
     uint32_t* fail_d = nullptr;
     uint32_t* pc_d = nullptr;
-    JobCounters* jc_h = nullptr;  // pinned mirror of the job counters after this pass
+    int cs = 0;  // chain stream of the pass in flight
     hipEvent_t eb0 = nullptr, eb1 = nullptr, ec0 = nullptr, ec1 = nullptr;
     std::vector<SegMeta> meta;
     uint32_t nsegs = 0, slots = 0;
@@ -215,7 +217,9 @@ struct PassBuf {
 struct Engine {
     int device = 0;
     ck_engine_config cfg{};
-    hipStream_t bstream = nullptr, cstream = nullptr;
+    hipStream_t bstream = nullptr, cstream[2] = {nullptr, nullptr};
+    uint64_t pass_seq = 0;
+    std::vector<JobCounters> jc_h;  // host copy of the counter slots
     char* arena_base = nullptr;
     size_t arena_per = 0;
     uint32_t* te0_d = nullptr;
@@ -425,7 +429,7 @@ template <int K>
 static void launch_chain(Engine& E, const PassBuf& B, bool bgh, bool desc) {
     const dim3 grid((B.nsegs + 3) / 4), block(256);
 #define CH(BG, M)                                                                                                \
-    hipLaunchKernelGGL((k_chain_mix<K, BG, M>), grid, block, 0, E.cstream, B.segs_d, B.nsegs, B.arena, B.walks, \
+    hipLaunchKernelGGL((k_chain_mix<K, BG, M>), grid, block, 0, E.cstream[B.cs], B.segs_d, B.nsegs, B.arena, B.walks, \
                        B.rnds, E.jc_d, B.pc_d, B.fail_d)
     if (bgh && desc)
         CH(true, uint64_t);
@@ -452,18 +456,18 @@ static int enqueue_pass(Engine& E, int bi) {
     }
     EH(hipGetLastError());
     EH(hipEventRecord(B.eb1, E.bstream));
-    EH(hipStreamWaitEvent(E.cstream, B.eb1, 0));
-    EH(hipMemsetAsync(B.pc_d, 0, PC_WORDS * 4, E.cstream));
-    EH(hipEventRecord(B.ec0, E.cstream));
+    B.cs = static_cast<int>(E.pass_seq++ & 1);
+    hipStream_t cst = E.cstream[B.cs];
+    EH(hipStreamWaitEvent(cst, B.eb1, 0));
+    EH(hipMemsetAsync(B.pc_d, 0, PC_WORDS * 4, cst));
+    EH(hipEventRecord(B.ec0, cst));
     switch (lead.spec.p.k) {
         case 2: launch_chain<2>(E, B, lead.bgh, lead.desc); break;
         case 3: launch_chain<3>(E, B, lead.bgh, lead.desc); break;
         default: launch_chain<4>(E, B, lead.bgh, lead.desc); break;
     }
     EH(hipGetLastError());
-    EH(hipMemcpyAsync(B.jc_h, E.jc_d, size_t(E.cfg.max_jobs) * sizeof(JobCounters), hipMemcpyDeviceToHost,
-                      E.cstream));
-    EH(hipEventRecord(B.ec1, E.cstream));
+    EH(hipEventRecord(B.ec1, cst));
     E.inflight.push_back(bi);
     return CK_OK;
 }
@@ -516,12 +520,21 @@ static int gather_pass(Engine& E, int bi) {
     if (B.weight > 0 && bms > 0) E.ms_per_bw = 0.7 * E.ms_per_bw + 0.3 * (bms / B.weight);
     if (probes > 0 && cms > 0) E.ms_per_probe = 0.7 * E.ms_per_probe + 0.3 * (cms / double(probes));
     for (const SegMeta& m : B.meta) m.job->completed += m.count;
+    // Passes finish in order here (each earlier one was gathered first), so the
+    // counters of a job whose last pass this is are final; the others are a
+    // running estimate for the scheduler.
+    int32_t lo = INT32_MAX, hi = -1;
+    for (const SegMeta& m : B.meta) {
+        lo = std::min(lo, m.job->cslot);
+        hi = std::max(hi, m.job->cslot);
+    }
+    EH(hipMemcpy(E.jc_h.data() + lo, E.jc_d + lo, size_t(hi - lo + 1) * sizeof(JobCounters), hipMemcpyDeviceToHost));
     Job* prev = nullptr;
     for (const SegMeta& m : B.meta) {
         Job* j = m.job;
         if (j == prev) continue;
         prev = j;
-        const JobCounters& c = B.jc_h[j->cslot];
+        const JobCounters& c = E.jc_h[j->cslot];
         if (c.trials) j->ppt = double(c.probes) / double(c.trials);
         if (!j->done && j->completed == j->spec.trials) finish_job(E, *j, c);
     }
@@ -536,7 +549,8 @@ static int admit(Engine& E) {
         E.pending.pop_front();
         j->cslot = E.free_cslots.back();
         E.free_cslots.pop_back();
-        EH(hipMemsetAsync(E.jc_d + j->cslot, 0, sizeof(JobCounters), E.cstream));
+        // On the build stream: every chain of the job waits for a build after it.
+        EH(hipMemsetAsync(E.jc_d + j->cslot, 0, sizeof(JobCounters), E.bstream));
         j->admitted = true;
         E.active.push_back(j);
     }
@@ -596,7 +610,8 @@ static void driver_main(Engine* ep) {
 static int engine_init(Engine& E) {
     EH(hipSetDevice(E.device));
     EH(hipStreamCreateWithFlags(&E.bstream, hipStreamNonBlocking));
-    EH(hipStreamCreateWithFlags(&E.cstream, hipStreamNonBlocking));
+    EH(hipStreamCreateWithFlags(&E.cstream[0], hipStreamNonBlocking));
+    EH(hipStreamCreateWithFlags(&E.cstream[1], hipStreamNonBlocking));
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, E.device) == hipSuccess && v > 0)
         E.lds_block = size_t(v);
@@ -611,6 +626,13 @@ static int engine_init(Engine& E) {
     EH(hipMemcpy(E.te0_d, te0(), 256 * sizeof(uint32_t), hipMemcpyHostToDevice));
     EH(hipMalloc(&E.jc_d, size_t(E.cfg.max_jobs) * sizeof(JobCounters)));
     EH(hipMemset(E.jc_d, 0, size_t(E.cfg.max_jobs) * sizeof(JobCounters)));
+    E.jc_h.assign(E.cfg.max_jobs, JobCounters{});
+    if (!E.cfg.arena_bytes) {
+        // Default: up to 192 GiB, leaving a sixth of the free memory to others.
+        size_t free_b = 0, total_b = 0;
+        EH(hipMemGetInfo(&free_b, &total_b));
+        E.cfg.arena_bytes = std::min<uint64_t>(uint64_t(192) << 30, free_b / 6 * 5);
+    }
     E.arena_per = (E.cfg.arena_bytes / E.cfg.buffers) & ~size_t(255);
     EH(hipMalloc(&E.arena_base, E.arena_per * E.cfg.buffers));
     const uint32_t maxseg = E.cfg.max_slots / SEG;
@@ -620,7 +642,6 @@ static int engine_init(Engine& E) {
         B.arena = E.arena_base + size_t(b) * E.arena_per;
         EH(hipMalloc(&B.segs_d, size_t(maxseg) * sizeof(SegDev)));
         EH(hipHostMalloc(&B.segs_h, size_t(maxseg) * sizeof(SegDev), hipHostMallocDefault));
-        EH(hipHostMalloc(&B.jc_h, size_t(E.cfg.max_jobs) * sizeof(JobCounters), hipHostMallocDefault));
         EH(hipMalloc(&B.walks, size_t(E.cfg.max_slots) * 8));
         EH(hipMalloc(&B.rnds, size_t(E.cfg.max_slots) * 8));
         EH(hipMalloc(&B.fail_d, size_t(E.cfg.max_slots) * 4));
@@ -640,7 +661,6 @@ static void engine_free(Engine& E) {
     for (PassBuf& B : E.bufs) {
         if (B.segs_d) (void)hipFree(B.segs_d);
         if (B.segs_h) (void)hipHostFree(B.segs_h);
-        if (B.jc_h) (void)hipHostFree(B.jc_h);
         if (B.walks) (void)hipFree(B.walks);
         if (B.rnds) (void)hipFree(B.rnds);
         if (B.fail_d) (void)hipFree(B.fail_d);
@@ -652,7 +672,8 @@ static void engine_free(Engine& E) {
     if (E.jc_d) (void)hipFree(E.jc_d);
     if (E.te0_d) (void)hipFree(E.te0_d);
     if (E.bstream) (void)hipStreamDestroy(E.bstream);
-    if (E.cstream) (void)hipStreamDestroy(E.cstream);
+    for (hipStream_t cs : E.cstream)
+        if (cs) (void)hipStreamDestroy(cs);
 }
 
 static int engine_error(Engine& E) { return fail(E.err, E.err_msg); }
@@ -674,8 +695,7 @@ int ck_engine_create(int device, const ck_engine_config* cfg, ck_engine** out) {
     *out = nullptr;
     ck_engine_config c{};
     if (cfg) c = *cfg;
-    if (!c.arena_bytes) c.arena_bytes = uint64_t(96) << 30;
-    if (!c.buffers) c.buffers = 3;
+    if (!c.buffers) c.buffers = 4;
     if (!c.max_slots) c.max_slots = 65536;
     if (!c.max_jobs) c.max_jobs = 4096;
     if (c.buffers < 2 || c.buffers > 8) return fail(CK_EINVAL, "buffers must be 2..8");
