@@ -120,38 +120,45 @@ def prewarm(ck, ctx, pts, ps):
     ctx.launch(ps[big], ck.batch_key(pts[big]["seed"], 0), 0, min(pts[big]["trials"], 1 << 16))
 
 
-def sweep_leg(ck, points: int, trials: int):
+def sweep_leg(ck, points: int, trials: int, plain_too: bool = True):
     """Mixed sweep (seed 2): one engine run over all points against one
     single-shape launch per point, same trials; failure counts must agree."""
     sw = load_sweep()
     pts = sw.sweep_points(points, seed=2, trials=trials)
     ps = [sw.params_of(ck, pt) for pt in pts]
-    ctx = ck.Context(0)
-    prewarm(ck, ctx, pts, ps)
-    t0 = time.perf_counter()
-    plain = []
-    for pt, p in zip(pts, ps):
-        f, st = ctx.launch(p, ck.batch_key(pt["seed"], 0), 0, pt["trials"])
-        plain.append((len(f), st.probes))
-    plain_s = time.perf_counter() - t0
-    ctx.close()
-    print(f"sweep: per-point launches {plain_s:.2f} s", file=sys.stderr, flush=True)
+    plain, plain_s = None, None
+    if plain_too:
+        ctx = ck.Context(0)
+        prewarm(ck, ctx, pts, ps)
+        t0 = time.perf_counter()
+        plain = []
+        for pt, p in zip(pts, ps):
+            f, st = ctx.launch(p, ck.batch_key(pt["seed"], 0), 0, pt["trials"])
+            plain.append((len(f), st.probes))
+        plain_s = time.perf_counter() - t0
+        ctx.close()
+        print(f"sweep: per-point launches {plain_s:.2f} s", file=sys.stderr, flush=True)
     eng = ck.Engine(0, policy=ck.SCHED_BALANCE)
     t0 = time.perf_counter()
     ids = [eng.submit(p, pt["seed"], pt["trials"], bt=pt["bt"]) for pt, p in zip(pts, ps)]
-    res = [eng.wait(i) for i in ids]
+    res = []
+    for n, i in enumerate(ids):
+        res.append(eng.wait(i))
+        if (n + 1) % 64 == 0:
+            print(f"sweep: {n + 1} jobs done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     eng_s = time.perf_counter() - t0
     st = eng.stats()
     eng.close()
     print(f"sweep: engine {eng_s:.2f} s", file=sys.stderr, flush=True)
-    equal = all((r.failed, r.probes) == pl for r, pl in zip(res, plain))
+    equal = plain is None or all((r.failed, r.probes) == pl for r, pl in zip(res, plain))
     total = sum(pt["trials"] for pt in pts)
     return {
         "workload": f"{points} points from seed 2 (n log-uniform [2^8,2^16], k 2..4, m/n U[1.2,1.6], stash 0..4, "
                     f"kick 100/500/1000, bgh25, ell_i 16), {trials} trials each",
         "jobs_per_s": points / eng_s, "trials_per_s": total / eng_s,
-        "plain_jobs_per_s": points / plain_s, "plain_trials_per_s": total / plain_s,
-        "speedup_vs_plain": plain_s / eng_s,
+        "plain_jobs_per_s": points / plain_s if plain_s else None,
+        "plain_trials_per_s": total / plain_s if plain_s else None,
+        "speedup_vs_plain": plain_s / eng_s if plain_s else None,
         "engine_s": eng_s, "plain_s": plain_s, "passes": st.passes,
         "device_build_ms": st.build_ms, "device_chain_ms": st.chain_ms,
         "failed": sum(r.failed for r in res), "probes": sum(r.probes for r in res),
@@ -207,6 +214,27 @@ def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float):
                         f"[{budget_lo:g}, {budget_hi:g}]", "rates": out}, ok
 
 
+def hbm_leg(ck, trials: int, layout: int = 0):
+    """Rare-event HBM shape: n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
+    ell_i 20, seed 4 -- the first `trials` trials of batch 0."""
+    p = ck.params("bgh25", 1 << 20, 20, k=3, buckets=1363149, kick=1000, stash=2)
+    ctx = ck.Context(0)
+    ctx.set_layout(layout)
+    ctx.launch(p, ck.batch_key(4, 0), 0, 64)  # workspace allocation outside the timed region
+    t0 = time.perf_counter()
+    f, st = ctx.launch(p, ck.batch_key(4, 0), 0, trials)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    chain_s = st.chain_ms / 1e3
+    return {
+        "workload": f"bgh25 n=2^20 m=1363149 k=3 stash=2 kick=1000 ell_i=20 seed=4, trials [0, {trials}) of batch 0",
+        "trials_per_s": trials / dt, "probes_per_s": st.probes / dt, "probes_per_trial": st.probes / trials,
+        "failed": len(f), "trials_per_pass": st.chunk,
+        "chain_algorithmic_gbs": SLOT_BYTES * st.probes / chain_s / 1e9 if chain_s > 0 else None,
+        "device_ms": {"build": st.build_ms, "chain": st.chain_ms},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,17 +244,22 @@ def main():
                     "the default is one whole batch")
     ap.add_argument("--tier", choices=["auto", "hbm", "lds"], default="auto")
     ap.add_argument("--chunk", type=int, default=0, help="trials per device pass (0 = library default)")
-    ap.add_argument("--layout", choices=["auto", "map", "desc", "map-xchg", "desc-plain"], default="auto")
+    ap.add_argument("--layout", choices=["auto", "map", "desc", "map-xchg", "desc-plain", "residual"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-points", type=int, default=512)
     ap.add_argument("--sweep-trials", type=int, default=1 << 15,
                     help="trials per sweep point (the full sweep is 10^6)")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-no-plain", action="store_true", help="skip the per-point baseline of the sweep")
     ap.add_argument("--serve-rates", default="10,100,1000")
     ap.add_argument("--serve-jobs", type=int, default=100)
     ap.add_argument("--serve-budget", default="10,1e5", help="trial budget range lo,hi (log-uniform)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--hbm-trials", type=int, default=28672,
+                    help="trials of the n=2^20 leg (default: two full memory-capped passes)")
+    ap.add_argument("--no-hbm", action="store_true")
+    ap.add_argument("--hbm-only", action="store_true", help="run only the n=2^20 leg (for counter profiles)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before warmup, copy a buffer of this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -243,14 +276,18 @@ def main():
     from paths import load_product
 
     ck = load_product()
+    layout = {"auto": ck.LAYOUT_AUTO, "map": ck.LAYOUT_MAP, "desc": ck.LAYOUT_DESC, "map-xchg": ck.LAYOUT_MAP_XCHG,
+              "desc-plain": ck.LAYOUT_DESC_PLAIN, "residual": ck.LAYOUT_RESIDUAL}[args.layout]
+    if args.hbm_only:
+        print(json.dumps({"hbm_tier": hbm_leg(ck, args.hbm_trials, layout)}), flush=True)
+        return 0
     p = ck.params(SHAPE["mode"], SHAPE["pnum"], SHAPE["ell"], kick=SHAPE["kick"])
     ctx = ck.Context(0)
     ctx.self_test()
     ctx.set_tier({"auto": ck.TIER_AUTO, "hbm": ck.TIER_HBM, "lds": ck.TIER_LDS}[args.tier])
     if args.chunk:
         ctx.set_chunk(args.chunk)
-    ctx.set_layout({"auto": ck.LAYOUT_AUTO, "map": ck.LAYOUT_MAP, "desc": ck.LAYOUT_DESC,
-                    "map-xchg": ck.LAYOUT_MAP_XCHG, "desc-plain": ck.LAYOUT_DESC_PLAIN}[args.layout])
+    ctx.set_layout(layout)
 
     if args.copy_probe_mib:
         n = args.copy_probe_mib << 18  # float32 elements
@@ -339,8 +376,11 @@ def main():
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)
     ctx.close()
+    if not args.no_hbm:
+        rec["hbm_tier"] = hbm_leg(ck, args.hbm_trials, layout)
+        print(f"hbm leg: {rec['hbm_tier']['trials_per_s']:.1f} trials/s", file=sys.stderr, flush=True)
     if not args.no_sweep:
-        rec["sweep"], good = sweep_leg(ck, args.sweep_points, args.sweep_trials)
+        rec["sweep"], good = sweep_leg(ck, args.sweep_points, args.sweep_trials, not args.sweep_no_plain)
         ok = ok and good
     if not args.no_serve:
         lo, hi = (float(x) for x in args.serve_budget.split(","))

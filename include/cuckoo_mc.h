@@ -52,12 +52,15 @@ enum {
 
 /* HBM-tier table layout and probe form. */
 enum {
-    CK_LAYOUT_AUTO = 0,       /* DESC_PLAIN when it fits and tables are not requested, else MAP_XCHG
-                                 below 2^14 trials per pass and MAP above */
+    CK_LAYOUT_AUTO = 0,       /* DESC_PLAIN when it fits and tables are not requested; else RESIDUAL
+                                 when memory caps the pass (single-buffered); else MAP_XCHG below 2^14
+                                 trials per pass and MAP above */
     CK_LAYOUT_MAP = 1,        /* entry = ordinal | choice (u16 when it fits); load + store probes */
     CK_LAYOUT_DESC = 2,       /* entry = occupant's packed candidates | choice (k*bits+2 <= 63); exchange probes */
     CK_LAYOUT_MAP_XCHG = 3,   /* u32 entry = ordinal | choice; exchange probes */
-    CK_LAYOUT_DESC_PLAIN = 4  /* descriptor entries; load + store probes */
+    CK_LAYOUT_DESC_PLAIN = 4, /* descriptor entries; load + store probes */
+    CK_LAYOUT_RESIDUAL = 5    /* entry = occupant's other k-1 candidates | choice (u32 when it fits, else u64),
+                                 its current bucket implied by the slot; load + store probes */
 };
 
 typedef struct ck_stats {

@@ -282,6 +282,175 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
 }
 
 // ---------------------------------------------------------------------------
+// Residual layout: a table entry holds the occupant's other k-1 bucket indices
+// (in candidate order, skipping its current one) and its choice,
+//   others | choice << (k-1)*bb,
+// and the occupant's current bucket is the slot the entry sits in.  An
+// eviction therefore rebuilds the displaced item's whole candidate list from
+// the entry it just read and the slot it read it from.  (k-1)*bb + 2 bits fit
+// a u32 entry at n = 2^14, k = 3 (2*15 + 2 = 32, choice <= 2 keeps the entry
+// below all-ones) and a u64 entry up to n = 2^20 and beyond, where the full
+// descriptor (k*bb + 2 bits) no longer fits.  New items take their candidates
+// from a stream in insertion order: one packed u64 per item when k*bb <= 64,
+// else the item's k map entries; it is prefetched one item ahead.
+template <int K, class E>
+struct ResCodec {
+    uint32_t bb;     // bits per bucket index
+    uint32_t cs;     // choice shift = (K-1)*bb
+    uint64_t bmask;  // (1 << bb) - 1
+    __device__ __forceinline__ E enc(const uint32_t (&c)[K], uint32_t w) const {
+        uint64_t e = uint64_t(w) << cs;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t sh = (uint32_t(i) - (uint32_t(i) > w ? 1u : 0u)) * bb;
+            e |= (uint32_t(i) == w) ? 0ull : (uint64_t(c[i]) << sh);
+        }
+        return static_cast<E>(e);
+    }
+    // Candidates of the occupant of bucket b; returns its choice.
+    __device__ __forceinline__ uint32_t dec(E e, uint32_t b, uint32_t (&c)[K]) const {
+        const uint64_t v = e;
+        const uint32_t w = static_cast<uint32_t>(v >> cs) & 3u;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t sh = (uint32_t(i) - (uint32_t(i) > w ? 1u : 0u)) * bb;
+            c[i] = (uint32_t(i) == w) ? b : static_cast<uint32_t>((v >> sh) & bmask);
+        }
+        return w;
+    }
+};
+
+// Candidate stream of new items: M = u64 is one packed descriptor per item
+// (bucket w at bits [w*bb, (w+1)*bb)), otherwise K map entries per item.
+template <int K, class M>
+struct CandStream {
+    const M* m;
+    uint32_t bb;
+    uint64_t bmask;
+    __device__ __forceinline__ void load(uint32_t item, uint32_t (&c)[K]) const {
+        if constexpr (sizeof(M) == 8) {
+            const uint64_t d = m[item];
+#pragma unroll
+            for (int w = 0; w < K; ++w) c[w] = static_cast<uint32_t>((d >> (w * bb)) & bmask);
+        } else {
+#pragma unroll
+            for (int w = 0; w < K; ++w) c[w] = m[item * K + w];
+        }
+    }
+};
+
+// BGH25 over the residual layout (same decisions as chain_bgh).
+template <int K, class E, class M>
+__device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
+                                              E* __restrict__ t, uint64_t walk, uint64_t rnd, uint32_t& probes) {
+    constexpr E kEmpty = static_cast<E>(~E(0));
+    uint32_t item = 0, coll = 0, stash_left = S.stash, pr = 0;
+    uint32_t c[K], nx[K];
+    st.load(0, c);
+    if (S.pnum > 1) st.load(1, nx);
+    bool failed = false;
+    while (item < S.pnum) {
+        if (rnd < K) rnd = splitmix(walk);
+        const uint32_t which = take_choice<K>(rnd);
+        const uint32_t b = pick<K>(c, which);
+        const E old = t[b];
+        ++pr;
+        bool advance = old == kEmpty;
+        if (!advance && ++coll > S.kick) {
+            if (!stash_left) {
+                failed = true;
+                break;
+            }
+            --stash_left;  // the item in hand goes to the stash; nothing is written
+            advance = true;
+        } else {
+            t[b] = R.enc(c, which);
+        }
+        if (advance) {
+            ++item;
+#pragma unroll
+            for (int w = 0; w < K; ++w) c[w] = nx[w];
+            if (item + 1 < S.pnum) st.load(item + 1, nx);
+            coll = 0;
+        } else {
+            R.dec(old, b, c);
+        }
+    }
+    probes = pr;
+    return failed;
+}
+
+// DRRT18 over the residual layout (same decisions as chain_drrt).
+template <int K, class E, class M>
+__device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
+                                               E* __restrict__ t, uint32_t& probes) {
+    constexpr E kEmpty = static_cast<E>(~E(0));
+    uint32_t pr = 0, stash_left = S.stash;
+    bool failed = false;
+    uint32_t nx[K];
+    st.load(0, nx);
+    for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
+        uint32_t c[K];
+#pragma unroll
+        for (int w = 0; w < K; ++w) c[w] = nx[w];
+        if (item + 1 < S.pnum) st.load(item + 1, nx);
+        bool placed = false;
+#pragma unroll
+        for (int w = 0; w < K; ++w) {
+            if (placed) break;
+            ++pr;
+            if (t[c[w]] == kEmpty) {
+                t[c[w]] = R.enc(c, uint32_t(w));
+                placed = true;
+            }
+        }
+        if (placed) continue;
+        uint32_t choice = 0, target = c[0], kicks = 0;
+        for (;;) {
+            const E disp = t[target];
+            ++pr;
+            if (disp == kEmpty) {
+                t[target] = R.enc(c, choice);
+                break;
+            }
+            if (kicks >= S.kick) {
+                if (stash_left) {
+                    --stash_left;
+                    break;
+                }
+                failed = true;
+                break;
+            }
+            t[target] = R.enc(c, choice);
+            choice = R.dec(disp, target, c);
+            const uint32_t prev = target;
+            bool found = false;
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+                if (found) break;
+                choice = (choice + 1 == K) ? 0 : choice + 1;
+                const uint32_t nb = pick<K>(c, choice);
+                if (nb != prev) {
+                    target = nb;
+                    found = true;
+                }
+            }
+            if (!found) {
+                if (stash_left) {
+                    --stash_left;
+                    break;
+                }
+                failed = true;
+                break;
+            }
+            ++kicks;
+        }
+    }
+    probes = pr;
+    return failed;
+}
+
+// ---------------------------------------------------------------------------
 // One-round-trip probes.  On this GPU family vector-memory loads and stores
 // retire in order under one counter, so a probe written as load + store makes
 // the next probe's load wait for the store's acknowledgement as well: two memory

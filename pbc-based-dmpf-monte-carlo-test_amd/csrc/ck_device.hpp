@@ -9,6 +9,14 @@
 
 namespace ck {
 
+// Register budgets that let a build block and chain waves share a CU (the HBM
+// tier overlaps the build of pass i+1 with the chains of pass i).  A SIMD has
+// 512 VGPRs per lane; a 16-wave build block puts 4 waves on each SIMD, so at
+// 96 VGPRs a wave it leaves 128 for chain waves (20-38 VGPRs each).  At 128
+// VGPRs the build block fills the register file, no chain wave can share its
+// CU, and the overlap collapses (measured: 1.7x slower passes at n = 2^14).
+#define CK_BUILD_REGS __attribute__((amdgpu_waves_per_eu(5)))
+
 constexpr uint32_t EMPTY = 0xffffffffu;
 constexpr uint32_t ORD_MASK = 0x3fffffffu;  // table entry = ordinal | choice << 30
 

@@ -68,7 +68,7 @@ constexpr uint32_t SEG = 64;
 
 // build: one wave per trial slot (slot = segment*64 + i).
 template <int K, bool CACHED, class M>
-__global__ __launch_bounds__(1024) void k_build_mix(const SegDev* __restrict__ segs, uint32_t slots,
+__global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build_mix(const SegDev* __restrict__ segs, uint32_t slots,
                                                    const uint32_t* __restrict__ te0, uint32_t tab_shift,
                                                    uint32_t cache_shift, char* __restrict__ arena,
                                                    uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds) {
@@ -82,11 +82,8 @@ __global__ __launch_bounds__(1024) void k_build_mix(const SegDev* __restrict__ s
     const uint32_t i = slot % SEG;
     if (i >= G.count) return;
     const Shape S = G.S;
-    RoundKeys bk;
-#pragma unroll
-    for (int w = 0; w < 44; ++w) bk.w[w] = G.bk.w[w];
     TrialSeed ts;
-    derive_uniform(T, bk, S, G.first + i, ts);
+    derive_uniform(T, G.bk, S, G.first + i, ts);  // batch key words read from the segment as used
 
     M* m = reinterpret_cast<M*>(arena + G.map_off + size_t(i) * G.map_bytes);
     if constexpr (CACHED) {

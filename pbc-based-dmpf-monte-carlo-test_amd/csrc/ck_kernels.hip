@@ -50,7 +50,7 @@ __device__ __forceinline__ LdsTab stage_tab(uint32_t* lds, uint32_t shift) {
 // descriptor layout (one packed candidate list per ordinal); the slot's table
 // (table_bytes, a multiple of 16) is filled with all-ones (empty).
 template <int K, bool CACHED, class M>
-__global__ __launch_bounds__(1024) void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
+__global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
                                                uint32_t tab_shift, M* __restrict__ map, size_t map_stride,
                                                char* __restrict__ tables, size_t table_bytes,
                                                uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
@@ -60,7 +60,6 @@ __global__ __launch_bounds__(1024) void k_build(Shape S, RoundKeys bk, uint32_t 
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t slot = blockIdx.x * waves + wave;
     if (slot >= count) return;  // wave-uniform; no block barrier follows
-
     TrialSeed ts;
     derive_uniform(T, bk, S, first + slot, ts);
     if (key_out && lane < 44) {
@@ -171,6 +170,55 @@ __global__ __launch_bounds__(256) void k_digest(Shape S, uint32_t count, const M
 #pragma unroll
             for (int w = 0; w < K; ++w) h += cand_hash_term(w, m[o * K + w]);
         }
+        sum += slot_digest(b, h, choice);
+    }
+    out[slot] = sum;
+}
+
+// chain (HBM tier, residual layout): M is the candidate stream (u64 packed
+// descriptor or u32 map rows), E the residual table entry (u32 or u64).
+template <int K, bool BGH, class M, class E>
+__global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint32_t count,
+                                                   const M* __restrict__ map, size_t map_stride,
+                                                   char* __restrict__ tables, size_t table_bytes,
+                                                   const uint64_t* __restrict__ walks,
+                                                   const uint64_t* __restrict__ rnds, uint32_t* __restrict__ counters,
+                                                   uint32_t* __restrict__ fail_offsets,
+                                                   uint32_t* __restrict__ probes_out) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool failed = false;
+    uint32_t pr = 0;
+    if (slot < count) {
+        const ResCodec<K, E> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+        const CandStream<K, M> st{map + slot * map_stride, S.dbits, (1ull << S.dbits) - 1};
+        E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
+        if constexpr (BGH)
+            failed = chain_bgh_res<K>(S, R, st, t, walks[slot], rnds[slot], pr);
+        else
+            failed = chain_drrt_res<K>(S, R, st, t, pr);
+        if (probes_out) probes_out[slot] = pr;
+    }
+    append_failures(failed, first + slot, counters + CNT_FAIL, fail_offsets);
+    add_wave_total(pr, reinterpret_cast<unsigned long long*>(counters + CNT_PROBES));
+}
+
+// Debug: per-slot digest of a residual-layout table.
+template <int K, class E>
+__global__ __launch_bounds__(256) void k_digest_res(Shape S, uint32_t count, const char* __restrict__ tables,
+                                                    size_t table_bytes, unsigned long long* __restrict__ out) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    const ResCodec<K, E> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+    const E* t = reinterpret_cast<const E*>(tables + slot * table_bytes);
+    uint64_t sum = 0;
+    for (uint32_t b = 0; b < S.buckets; ++b) {
+        const E e = t[b];
+        if (e == static_cast<E>(~E(0))) continue;
+        uint32_t c[K];
+        const uint32_t choice = R.dec(e, b, c);
+        uint64_t h = 0;
+#pragma unroll
+        for (int w = 0; w < K; ++w) h += cand_hash_term(w, c[w]);
         sum += slot_digest(b, h, choice);
     }
     out[slot] = sum;
@@ -297,7 +345,7 @@ struct Ctx {
     uint32_t tier = CK_TIER_AUTO;
     uint32_t layout = CK_LAYOUT_AUTO;
     uint32_t chunk_cap = 0;
-    size_t budget = size_t(96) << 30;  // HBM budget for the double-buffered maps and tables
+    size_t budget = size_t(96) << 30;  // HBM budget for the double-buffered maps and tables (set per call)
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
     uint32_t cus = 1;
@@ -386,12 +434,23 @@ static uint32_t choose_tier(const Ctx& c, const ck_params&) {
 // the top bit of a u64 clear (so all-ones stays the empty entry).
 static bool desc_ok(const ck_params& p) { return p.k * make_shape(p).dbits + 2 <= 63; }
 
+// Residual layout: (k-1)*bits + 2 must stay below an all-ones entry -- fewer
+// bits than the entry width, or exactly the width with a choice below 3.
+static bool res_fits(const ck_params& p, uint32_t width) {
+    const uint32_t need = (p.k - 1) * make_shape(p).dbits + 2;
+    return need < width || (need == width && p.k <= 3);
+}
+static bool res_ok(const ck_params& p) { return res_fits(p, 64); }
+static bool res_e32(const ck_params& p) { return res_fits(p, 32); }
+static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits <= 64; }
+
 // HBM-tier pass buffer: [counters | map C x map_bytes | tables C x table_bytes |
 // walk C | rnd C | probes C | fail C | keys C x 44 | digests C], twice
 // (double-buffered).  In the descriptor layout the "map" is one u64 candidate
 // descriptor per ordinal and the table entries are u64.
 struct HbmPlan {
     bool desc, map16, table16, xchg;
+    bool res, stream64, e32;  // residual layout: u64 candidate stream, u32 entries
     size_t map_stride;   // map entries per slot
     size_t map_bytes;    // per slot
     size_t table_bytes;  // per slot
@@ -402,10 +461,15 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout) {
     const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     h.desc = desc;
     h.xchg = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_MAP_XCHG;
-    h.map16 = !desc && map16_ok(p);
-    h.table16 = !desc && !h.xchg && table16_ok(p);
-    const size_t mw = desc ? 8 : h.map16 ? 2 : 4, tw = desc ? 8 : h.table16 ? 2 : 4;
-    h.map_bytes = align256(size_t(p.pnum) * (desc ? 1 : p.k) * mw);
+    h.res = layout == CK_LAYOUT_RESIDUAL;
+    h.stream64 = h.res && res_stream64(p);
+    h.e32 = h.res && res_e32(p);
+    h.map16 = !desc && !h.res && map16_ok(p);
+    h.table16 = !desc && !h.res && !h.xchg && table16_ok(p);
+    const bool m64 = desc || h.stream64;
+    const size_t mw = m64 ? 8 : h.map16 ? 2 : 4;
+    const size_t tw = h.res ? (h.e32 ? 4 : 8) : desc ? 8 : h.table16 ? 2 : 4;
+    h.map_bytes = align256(size_t(p.pnum) * (m64 ? 1 : p.k) * mw);
     h.map_stride = h.map_bytes / mw;
     h.table_bytes = align256(size_t(p.buckets) * tw);
     size_t o = 0;
@@ -438,12 +502,33 @@ static constexpr uint32_t kXchgBelow = 1u << 14;
 // Trials per pass: 65536 chains in flight saturate the random-access path
 // (measured: 32768 slower, 131072 and more no faster).
 static constexpr uint32_t kHbmChunk = 1u << 16;
-static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout) {
+static size_t slot_bytes(const ck_params& p, uint32_t layout) {
     const HbmPlan one = plan_hbm(p, 1, layout);
-    const size_t per_slot = one.map_bytes + one.table_bytes + 8 + 8 + 4 + 4 + 176;
-    const size_t fit = std::max<size_t>(1, c.budget / (2 * per_slot));
+    return one.map_bytes + one.table_bytes + 8 + 8 + 4 + 4 + 176;
+}
+// Pass buffers: two, so the build of pass i+1 runs under the chain of pass i,
+// unless memory caps a pass below its target.  Then one buffer of twice the
+// slots: a memory-capped shape (n = 2^20: 17.5 MB per slot) is bound by the
+// latency of its chains, and its build is a few percent of the chain time.
+static uint32_t hbm_buffers(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout) {
+    const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
+    return c.budget / (2 * slot_bytes(p, layout)) < std::min<size_t>(target, count) ? 1 : 2;
+}
+static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout, uint32_t nbuf) {
+    const size_t fit = std::max<size_t>(1, c.budget / (nbuf * slot_bytes(p, layout)));
     const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
     return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>({fit, size_t(target), size_t(count)})));
+}
+
+// HBM budget of the two pass buffers: most of the free device memory (the
+// current workspace counts as free, since growing it frees it first), at most
+// 240 GiB.  Large shapes (n = 2^20: 17.5 MB per trial slot) are latency-bound,
+// so every extra slot is an extra chain in flight.
+static constexpr size_t kBudgetCap = size_t(240) << 30;
+static size_t hbm_budget(const Ctx& c) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return size_t(96) << 30;
+    return std::min(kBudgetCap, (free_b + c.ws.bytes) / 20 * 17);
 }
 
 // LDS-tier per-launch scratch: [counters | probes C | fail C | digests C | tables C x buckets].
@@ -512,7 +597,7 @@ static void launch_build_t(const BuildPlan& bp, const PassArgs& a, hipStream_t s
 }
 template <int K>
 static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs& a, hipStream_t st) {
-    if (h.desc)
+    if (h.desc || h.stream64)
         launch_build_t<K, uint64_t>(bp, a, st);
     else if (h.map16)
         launch_build_t<K, uint16_t>(bp, a, st);
@@ -534,7 +619,13 @@ static void launch_digest_t(const PassArgs& a, hipStream_t st) {
 }
 template <int K>
 static void launch_digest_k(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
-    if (h.desc)
+    if (h.res && h.e32)
+        hipLaunchKernelGGL((k_digest_res<K, uint32_t>), dim3((a.count + 255) / 256), dim3(256), 0, st, a.S, a.count,
+                           a.tables, a.table_bytes, a.digests);
+    else if (h.res)
+        hipLaunchKernelGGL((k_digest_res<K, uint64_t>), dim3((a.count + 255) / 256), dim3(256), 0, st, a.S, a.count,
+                           a.tables, a.table_bytes, a.digests);
+    else if (h.desc)
         launch_digest_t<K, uint64_t, uint64_t>(a, st);
     else if (h.map16 && h.table16)
         launch_digest_t<K, uint16_t, uint16_t>(a, st);
@@ -546,9 +637,24 @@ static void launch_digest_k(const HbmPlan& h, const PassArgs& a, hipStream_t st)
         launch_digest_t<K, uint32_t, uint32_t>(a, st);
 }
 
+template <int K, bool BGH, class M, class E>
+static void launch_chain_res_t(const PassArgs& a, hipStream_t st) {
+    const uint32_t blocks = (a.count + 255) / 256;
+    hipLaunchKernelGGL((k_chain_res<K, BGH, M, E>), dim3(blocks), dim3(256), 0, st, a.S, a.first, a.count,
+                       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds,
+                       a.cnt, a.fail, a.probes);
+}
 template <int K, bool BGH>
 static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
-    if (h.desc && h.xchg)
+    if (h.res && h.stream64 && h.e32)
+        launch_chain_res_t<K, BGH, uint64_t, uint32_t>(a, st);
+    else if (h.res && h.stream64)
+        launch_chain_res_t<K, BGH, uint64_t, uint64_t>(a, st);
+    else if (h.res && h.e32)
+        launch_chain_res_t<K, BGH, uint32_t, uint32_t>(a, st);
+    else if (h.res)
+        launch_chain_res_t<K, BGH, uint32_t, uint64_t>(a, st);
+    else if (h.desc && h.xchg)
         launch_chain_t<K, BGH, uint64_t, uint64_t, true>(a, st);
     else if (h.desc)
         launch_chain_t<K, BGH, uint64_t, uint64_t, false>(a, st);
@@ -676,6 +782,7 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
 // and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
                    ck_stats* st) {
+    c.budget = hbm_budget(c);
     // Auto: descriptor entries with load + store probes where they fit (fastest
     // with tens of thousands of chains in flight); otherwise ordinal entries,
     // with exchange probes when few chains are in flight (one round trip per
@@ -684,20 +791,32 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     if (layout == CK_LAYOUT_AUTO) {
         if (!out.tables && desc_ok(p))
             layout = CK_LAYOUT_DESC_PLAIN;
+        else if (!out.tables && res_ok(p) && hbm_buffers(c, p, count, CK_LAYOUT_RESIDUAL) == 1)
+            // Memory-capped, single-buffered passes (n = 2^20): one random read
+            // per probe beats exchange probes plus a map read per eviction
+            // (measured 2416 vs 2170 trials/s).
+            layout = CK_LAYOUT_RESIDUAL;
         else
-            layout = hbm_chunk(c, p, count, CK_LAYOUT_MAP_XCHG) < kXchgBelow ? CK_LAYOUT_MAP_XCHG : CK_LAYOUT_MAP;
+            layout = hbm_chunk(c, p, count, CK_LAYOUT_MAP_XCHG, hbm_buffers(c, p, count, CK_LAYOUT_MAP_XCHG)) < kXchgBelow
+                         ? CK_LAYOUT_MAP_XCHG
+                         : CK_LAYOUT_MAP;
     }
     const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     if (desc && !desc_ok(p)) return fail(CK_EINVAL, "descriptor layout needs k*bits(buckets-1) + 2 <= 63");
     if (desc && out.tables) return fail(CK_EINVAL, "the descriptor layout does not keep ordinals; use the map layout for tables");
-    const uint32_t chunk = hbm_chunk(c, p, count, layout);
+    if (layout == CK_LAYOUT_RESIDUAL && !res_ok(p))
+        return fail(CK_EINVAL, "residual layout needs (k-1)*bits(buckets-1) + 2 <= 63");
+    if (layout == CK_LAYOUT_RESIDUAL && out.tables)
+        return fail(CK_EINVAL, "the residual layout does not keep ordinals; use the map layout for tables");
+    const uint32_t nbuf = hbm_buffers(c, p, count, layout);
+    const uint32_t chunk = hbm_chunk(c, p, count, layout, nbuf);
     const HbmPlan h = plan_hbm(p, chunk, layout);
-    int rc = ensure_ws(c, 2 * h.buffer);
+    int rc = ensure_ws(c, nbuf * h.buffer);
     if (rc) return rc;
     const BuildPlan bp = plan_build(p, c.lds_block);
     const bool bgh = p.mode == CK_MODE_BGH25;
     PassArgs args[2];
-    for (int b = 0; b < 2; ++b) {
+    for (uint32_t b = 0; b < nbuf; ++b) {
         char* base = static_cast<char*>(c.ws.base) + b * h.buffer;
         PassArgs& a = args[b];
         a.S = make_shape(p);
@@ -720,7 +839,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     }
     const uint32_t passes = (count + chunk - 1) / chunk;
     auto enqueue = [&](uint32_t i) -> int {
-        const int b = i & 1;
+        const int b = nbuf == 2 ? int(i & 1) : 0;
         PassArgs& a = args[b];
         a.first = first + i * chunk;
         a.count = std::min(chunk, count - i * chunk);
@@ -754,7 +873,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         return CK_OK;
     };
     auto gather = [&](uint32_t i) -> int {
-        const int b = i & 1;
+        const int b = nbuf == 2 ? int(i & 1) : 0;
         const PassArgs& a = args[b];
         CKH(hipEventSynchronize(c.ev_chain[b][1]));
         if (a.digests) CKH(hipStreamSynchronize(c.stream));
@@ -765,11 +884,12 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         if (out.tables &&
             (r = copy_rows(out.tables + done * p.buckets, a.tables, h.table_bytes, p.buckets, a.count, h.table16, true)))
             return r;
-        if (out.map && !h.desc &&
+        const bool map64 = h.desc || h.stream64;
+        if (out.map && !map64 &&
             (r = copy_rows(out.map + done * p.pnum * p.k, a.map, h.map_bytes, size_t(p.pnum) * p.k, a.count, h.map16,
                            false)))
             return r;
-        if (out.map && h.desc) {
+        if (out.map && map64) {
             std::vector<uint64_t> d(size_t(p.pnum) * a.count);
             CKH(hipMemcpy2D(d.data(), size_t(p.pnum) * 8, a.map, h.map_bytes, size_t(p.pnum) * 8, a.count,
                             hipMemcpyDeviceToHost));
@@ -793,6 +913,11 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         }
         return CK_OK;
     };
+    if (nbuf == 1) {
+        for (uint32_t i = 0; i < passes; ++i)
+            if ((rc = enqueue(i)) || (rc = gather(i))) return rc;
+        return CK_OK;
+    }
     if ((rc = enqueue(0))) return rc;
     for (uint32_t i = 1; i < passes; ++i) {
         if ((rc = enqueue(i))) return rc;
@@ -1006,7 +1131,7 @@ int ck_launch_digest(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uin
 }
 
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout) {
-    if (!ctx || layout > CK_LAYOUT_DESC_PLAIN) return fail(CK_EINVAL, "bad layout");
+    if (!ctx || layout > CK_LAYOUT_RESIDUAL) return fail(CK_EINVAL, "bad layout");
     ctx->c.layout = layout;
     return CK_OK;
 }

@@ -20,7 +20,7 @@ DERIVED = 0xFFFFFFFF
 TIER_AUTO, TIER_HBM, TIER_LDS = 0, 1, 2
 PENDING = 1
 SCHED_BALANCE, SCHED_FIFO, SCHED_SRPT = 0, 1, 2
-LAYOUT_AUTO, LAYOUT_MAP, LAYOUT_DESC, LAYOUT_MAP_XCHG, LAYOUT_DESC_PLAIN = 0, 1, 2, 3, 4
+LAYOUT_AUTO, LAYOUT_MAP, LAYOUT_DESC, LAYOUT_MAP_XCHG, LAYOUT_DESC_PLAIN, LAYOUT_RESIDUAL = 0, 1, 2, 3, 4, 5
 MODES = {"bgh25": BGH25, "drrt18": DRRT18}
 
 

@@ -114,3 +114,42 @@ def test_cli_extension_journal(tmp_path):
         f, _, _ = ock.run_range(ock.batch_key(9, b), 0, n, op)
         want += [b * 256 + int(x) for x in f]
     assert failures(d) == want
+
+
+def test_cli_checkpoint_resume_n2_20(tmp_path):
+    """The rare-event HBM shape (n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
+    seed 4) through the journal: a run paused after its first committed batch
+    and resumed with two workers ends with the same summary and failures as an
+    uninterrupted run."""
+    shape = ["--mode", "bgh25", "--pnum", str(1 << 20), "--ell-i", "20", "--k", "3", "--buckets", "1363149",
+             "--kick", "1000", "--stash", "2", "--seed", "4", "--trials", "1536", "--bt", "9", "--micro", "512"]
+    whole = tmp_path / "whole"
+    run([*shape, "--workers", "1", "--out", str(whole)], timeout=300)
+    d = tmp_path / "resumed"
+    d.mkdir()
+    proc = subprocess.Popen([str(CLI), *shape, "--workers", "1", "--out", str(d)], stdout=subprocess.PIPE,
+                            stderr=subprocess.PIPE, text=True)
+    try:
+        deadline = time.monotonic() + 300
+        while proc.poll() is None:
+            if any(p.stat().st_size > 64 for p in d.glob("worker_*.bin")):
+                (d / "PAUSE").touch()
+                break
+            assert time.monotonic() < deadline
+            time.sleep(0.01)
+        proc.wait(timeout=300)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+            proc.wait()
+    assert proc.returncode == 0
+    first = summary(d)
+    (d / "PAUSE").unlink(missing_ok=True)
+    run([*shape, "--workers", "2", "--out", str(d)], timeout=300)
+    a, b = summary(whole), summary(d)
+    assert a["status"] == b["status"] == "complete"
+    assert a["completed"] == b["completed"] == 1536 and a["journal_version"] == 4
+    assert a["failed"] == b["failed"]
+    assert (whole / "failures.csv").read_text().splitlines()[1:] == (d / "failures.csv").read_text().splitlines()[1:]
+    if first["status"] == "paused":
+        assert 0 < first["completed"] < 1536 and first["completed"] % 512 == 0

@@ -182,7 +182,7 @@ def test_tiers_agree_on_mixed_shapes(ctx):
 
 COMBOS = {"hbm-map": (ck.TIER_HBM, ck.LAYOUT_MAP), "hbm-desc": (ck.TIER_HBM, ck.LAYOUT_DESC),
           "hbm-map-xchg": (ck.TIER_HBM, ck.LAYOUT_MAP_XCHG), "hbm-desc-plain": (ck.TIER_HBM, ck.LAYOUT_DESC_PLAIN),
-          "lds": (ck.TIER_LDS, ck.LAYOUT_AUTO)}
+          "hbm-residual": (ck.TIER_HBM, ck.LAYOUT_RESIDUAL), "lds": (ck.TIER_LDS, ck.LAYOUT_AUTO)}
 
 DIGEST_SHAPES = [
     ("bgh25", 16, 8, {}, 1024),
@@ -251,3 +251,53 @@ def test_descriptor_layout_rejects_oversized_shapes(ctx):
             ctx.launch(p, ck.batch_key(4, 0), 0, 1)
     finally:
         ctx.set_layout(ck.LAYOUT_AUTO)
+
+
+def test_n2_20_one_and_two_buffer_passes_agree(ctx):
+    """The rare-event HBM shape (n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
+    seed 4).  One 16384-trial call is memory-capped, so it runs single-buffered
+    passes; 4096-trial calls run double-buffered.  Failures and probe totals
+    agree, and the first 24 trials match the oracle probe for probe."""
+    p, op = both("bgh25", 1 << 20, 20, k=3, buckets=1363149, kick=1000, stash=2)
+    rk = ck.batch_key(4, 0)
+    f1, st1 = ctx.launch(p, rk, 0, 16384)
+    f2, probes2 = [], 0
+    for off in range(0, 16384, 4096):
+        f, st = ctx.launch(p, rk, off, 4096)
+        f2 += list(map(int, f))
+        probes2 += st.probes
+    assert list(map(int, f1)) == f2
+    assert st1.probes == probes2
+    f, pr, _, _ = ctx.launch_debug(p, rk, 0, 24)
+    of, opr, _ = ock.run_range(ock.batch_key(4, 0), 0, 24, op, probes=True)
+    assert list(f) == list(of) and (pr == opr).all()
+
+
+RESIDUAL_SHAPES = [
+    ("bgh25", 16384, 16, {"kick": 500}, 64),                                   # u32 entries, 2*15 + 2 = 32 bits
+    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50, "stash": 1}, 256),  # 3*10 + 2 = 32 bits, k = 4: u64
+    ("bgh25", 60000, 16, {"k": 4, "buckets": 70000, "kick": 200, "stash": 2}, 24),  # 4*17 > 64: u32 map stream
+    ("drrt18", 60000, 16, {"k": 4, "buckets": 70000, "kick": 100}, 24),
+    ("bgh25", 1 << 20, 20, {"buckets": 1363149, "kick": 1000, "stash": 2}, 6),   # n = 2^20: u64 entries
+]
+
+
+@pytest.mark.parametrize("mode,pnum,ell,kw,count", RESIDUAL_SHAPES)
+def test_residual_layout(ctx, mode, pnum, ell, kw, count):
+    """Residual entries (the occupant's other candidates and its choice) at the
+    entry-width boundaries and with both candidate-stream forms."""
+    p, op = both(mode, pnum, ell, **kw)
+    ctx.set_layout(ck.LAYOUT_RESIDUAL)
+    try:
+        f, probes, dig, st = ctx.launch_digest(p, ck.batch_key(21, 1), 5, count)
+        with pytest.raises(ck.CkError):  # ordinals are not kept
+            ctx.launch_debug(p, ck.batch_key(21, 1), 5, 1, tables=True)
+    finally:
+        ctx.set_layout(ck.LAYOUT_AUTO)
+    bk = ock.batch_key(21, 1)
+    of, oprobes, _ = ock.run_range(bk, 5, count, op, probes=True)
+    assert list(f) == list(of)
+    assert (probes == oprobes).all()
+    for i in range(0, count, max(1, count // 8)):
+        _, bins, _ = ock.trial_run(bk, 5 + i, op)
+        assert int(dig[i]) == ock.table_digest(bins, ock.bucket_map(bk, 5 + i, op), op.k), i
