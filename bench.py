@@ -88,12 +88,44 @@ def cpu_baseline(ctx, ck, p, seconds: float):
     dt = time.perf_counter() - t0
     gf, _ = ctx.launch(p, ck.batch_key(SEED, 0), 0, sample)
     parity = list(gf) == list(of)
+    one = max(4, sample // (cores * 2))
+    t0 = time.perf_counter()
+    ock.run_range(bk, 0, one, op, threads=1)
+    one_rate = one / (time.perf_counter() - t0)
+    g15, _ = ctx.launch(p, ck.batch_key(SEED, 0), 0, 1 << 15)
     return {
         "value": sample / dt, "unit": "trials/s", "cores": cores, "kind": "port",
         "sample": f"first {sample} trials of batch 0 (seed 1), oracle/ck_oracle.c with AES-NI round cache, "
                   f"{cores} threads on {cpu_model()}",
         "seconds": dt, "failures": len(of), "gpu_failures_equal": parity,
+        "single_core_trials_per_s": one_rate,
+        "reference_cpu_bench": cpu_reference(cores, len(g15)),
     }, parity
+
+
+def cpu_reference(cores: int, gpu_failures_prefix):
+    """The reference's own cpu_bench (oracle/_ref, built from its sources) on the
+    headline shape: first 2^15 trials of batch 0.  Its CLI derives the kick
+    bound from the mode (3n = 49152 rather than 500); no insertion at this load
+    comes near 500 kicks, so the work and the failures are the same."""
+    import re
+    import subprocess
+    import tempfile
+    exe = ROOT / "oracle" / "_ref" / "cpu_bench"
+    if not exe.exists():
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        out = subprocess.run([str(exe), "--mode", "bgh25", "--pnum", str(SHAPE["pnum"]), "--ell-i", str(SHAPE["ell"]),
+                              "--seed", str(SEED), "--et", "15", "--bt", "15", "--workers", str(cores), "--out", d],
+                             capture_output=True, text=True, timeout=600)
+    m = re.search(r"RESULT .*completed=(\d+) failed=(\d+) elapsed_s=([0-9.e+-]+) work_s=([0-9.e+-]+)", out.stdout)
+    if out.returncode != 0 or not m:
+        return {"error": out.stderr[-300:]}
+    done, failed, work = int(m.group(1)), int(m.group(2)), float(m.group(4))
+    return {"value": done / work, "unit": "trials/s", "cores": cores, "kind": "reference",
+            "sample": f"reference cpu_bench (oracle/_ref) bgh25 pnum 16384 ell_i 16 seed 1, trials 0..{done - 1} "
+                      f"of batch 0, kick 3n", "seconds": work, "failures": failed,
+            "gpu_failures_equal": failed == gpu_failures_prefix}
 
 
 def load_sweep():
@@ -141,11 +173,22 @@ def sweep_leg(ck, points: int, trials: int, plain_too: bool = True):
     eng = ck.Engine(0, policy=ck.SCHED_BALANCE)
     t0 = time.perf_counter()
     ids = [eng.submit(p, pt["seed"], pt["trials"], bt=pt["bt"]) for pt, p in zip(pts, ps)]
-    res = []
-    for n, i in enumerate(ids):
-        res.append(eng.wait(i))
-        if (n + 1) % 64 == 0:
-            print(f"sweep: {n + 1} jobs done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    # Jobs finish in the scheduler's order, not in submission order: poll all
+    # of them and report progress every 20 s.
+    res = [None] * len(ids)
+    left, last = len(ids), time.perf_counter()
+    while left:
+        for n, i in enumerate(ids):
+            if res[n] is None:
+                r = eng.poll(i)
+                if r is not None:
+                    res[n] = r
+                    left -= 1
+        if left:
+            time.sleep(0.005)
+        if time.perf_counter() - last > 20:
+            last = time.perf_counter()
+            print(f"sweep: {len(ids) - left} of {len(ids)} jobs done at {last - t0:.1f} s", file=sys.stderr, flush=True)
     eng_s = time.perf_counter() - t0
     st = eng.stats()
     eng.close()
@@ -214,6 +257,29 @@ def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float):
                         f"[{budget_lo:g}, {budget_hi:g}]", "rates": out}, ok
 
 
+def config1_leg(ck):
+    """The reference's CPU-runnable point: bgh25 n=1024 m=1536 k=3 stash 0 kick
+    500, ell_i 16, seed 0, trials [0, 100000) of batch 0 -- GPU and CPU oracle
+    (all host cores) on the whole range, failure lists compared."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ck_oracle as ock
+    n = 100000
+    p = ck.params("bgh25", 1024, 16, kick=500)
+    op = ock.parameters("bgh25", 1024, 16, kick=500)
+    with ck.Context(0) as ctx:
+        ctx.launch(p, ck.batch_key(0, 0), 0, 1024)
+        t0 = time.perf_counter()
+        f, st = ctx.launch(p, ck.batch_key(0, 0), 0, n)
+        gdt = time.perf_counter() - t0
+    cores = host_cores()
+    t0 = time.perf_counter()
+    of, _, _ = ock.run_range(ock.batch_key(0, 0), 0, n, op, threads=cores)
+    cdt = time.perf_counter() - t0
+    return {"workload": "bgh25 n=1024 m=1536 k=3 stash=0 kick=500 ell_i=16 seed=0, trials [0, 100000) of batch 0",
+            "gpu_trials_per_s": n / gdt, "cpu_trials_per_s": n / cdt, "cpu_cores": cores, "failed": len(f),
+            "failures_equal": list(f) == list(of)}, list(f) == list(of)
+
+
 def hbm_leg(ck, trials: int, layout: int = 0):
     """Rare-event HBM shape: n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
     ell_i 20, seed 4 -- the first `trials` trials of batch 0."""
@@ -259,6 +325,8 @@ def main():
     ap.add_argument("--hbm-trials", type=int, default=28672,
                     help="trials of the n=2^20 leg (default: two full memory-capped passes)")
     ap.add_argument("--no-hbm", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune-sample", type=int, default=1 << 17, help="trials per autotuner candidate")
     ap.add_argument("--hbm-only", action="store_true", help="run only the n=2^20 leg (for counter profiles)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before warmup, copy a buffer of this many MiB once (calibrates HBM byte counters)")
@@ -376,6 +444,19 @@ def main():
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)
     ctx.close()
+    if not args.no_tune:
+        with ck.Context(0) as tctx:
+            names = {0: "auto", 1: "map", 2: "desc", 3: "map-xchg", 4: "desc-plain", 5: "residual"}
+            ents = tctx.tune(p, ck.batch_key(SEED, 0), args.tune_sample)
+        rec["autotune"] = {"sample_trials": args.tune_sample, "candidates": [
+            {"tier": {1: "hbm", 2: "lds"}.get(e["tier"], "auto"), "layout": names[e["layout"]], "chunk": e["chunk"],
+             "trials_per_s": round(e["trials_per_s"])} for e in ents]}
+        best = max(ents, key=lambda e: e["trials_per_s"])
+        print(f"autotune: best tier {best['tier']} layout {names[best['layout']]} chunk {best['chunk']} "
+              f"{best['trials_per_s']:.0f} trials/s", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        rec["config1"], good = config1_leg(ck)
+        ok = ok and good
     if not args.no_hbm:
         rec["hbm_tier"] = hbm_leg(ck, args.hbm_trials, layout)
         print(f"hbm leg: {rec['hbm_tier']['trials_per_s']:.1f} trials/s", file=sys.stderr, flush=True)

@@ -101,6 +101,23 @@ int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
 /* Forces an HBM-tier table layout (CK_LAYOUT_*). */
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
 
+/* One autotuner measurement. */
+typedef struct ck_tune_entry {
+    uint32_t tier, layout, chunk; /* CK_TIER_*, CK_LAYOUT_*, trials per pass (0: library default) */
+    uint32_t pad;
+    double trials_per_s;          /* wall-clock rate on the sample */
+} ck_tune_entry;
+
+/* Autotuner (the north star's per-shape tuning of the placement that the
+ * reference exposes as --table-layout/--shared-variant/--block-threads,
+ * cuckoo_bench.cu:819-901): runs trials [0, sample) of the batch under every
+ * applicable tier / table layout / trials-per-pass candidate, checks that all
+ * of them report the same failing trials (CK_ERANGE otherwise), keeps the
+ * fastest as this context's setting and reports each measurement in
+ * entries[0 .. min(cap, *n)-1]. */
+int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t sample,
+                ck_tune_entry* entries, uint32_t cap, uint32_t* n);
+
 /* FIPS-197 AES-128 known answer through the device AES path
  * (aes_self_test / cached_aes_self_test, cuckoo_bench.cu:332-354). */
 int ck_aes_self_test(ck_ctx* ctx);

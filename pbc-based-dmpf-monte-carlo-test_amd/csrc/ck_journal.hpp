@@ -46,6 +46,7 @@ struct Options {
     uint32_t k = 3, buckets = CK_DERIVED, kick = CK_DERIVED, stash = 0;
     uint64_t trials = 0;  // 0: planned = 2^et
     std::string tier = "auto";
+    bool tune = false;  // --tune: each worker times the placements on a sample first
 };
 
 Options parse(int argc, char** argv);

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1128,6 +1129,68 @@ int ck_launch_digest(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uin
                      ck_stats* stats) {
     if (!digests) return fail(CK_EINVAL, "null digests");
     return launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, probes, nullptr, digests, stats);
+}
+
+// Autotuner.  Each applicable (tier, layout, trials per pass) candidate runs
+// the first sample/8 trials once (workspace growth, warm caches), then the
+// whole sample with wall-clock timing.  Every candidate must report the same
+// failing offsets; the fastest becomes the context's setting.
+int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t sample, ck_tune_entry* entries,
+                uint32_t cap, uint32_t* n) {
+    if (!ctx || !rk || !n || (cap && !entries)) return fail(CK_EINVAL, "null argument");
+    if (sample == 0) return fail(CK_EINVAL, "tuning needs at least one trial");
+    int rc = check_params(p);
+    if (rc) return rc;
+    Ctx& c = ctx->c;
+    struct Cand {
+        uint32_t tier, layout, chunk;
+    };
+    std::vector<Cand> cands;
+    const uint32_t half = std::max<uint32_t>(64, std::min<uint32_t>(sample, kHbmChunk) / 2);
+    for (uint32_t layout : {CK_LAYOUT_DESC_PLAIN, CK_LAYOUT_RESIDUAL, CK_LAYOUT_MAP, CK_LAYOUT_MAP_XCHG}) {
+        if (layout == CK_LAYOUT_DESC_PLAIN && !desc_ok(*p)) continue;
+        if (layout == CK_LAYOUT_RESIDUAL && !res_ok(*p)) continue;
+        cands.push_back({CK_TIER_HBM, layout, 0});
+        if (half < sample) cands.push_back({CK_TIER_HBM, layout, half});
+    }
+    if (plan_lds(*p, c.lds_block, c.lds_cu).ok) cands.push_back({CK_TIER_LDS, CK_LAYOUT_AUTO, 0});
+    const uint32_t saved_tier = c.tier, saved_layout = c.layout, saved_chunk = c.chunk_cap;
+    std::vector<uint32_t> want, got(sample);
+    bool have_want = false;
+    double best_rate = -1;
+    Cand best = {saved_tier, saved_layout, saved_chunk};
+    *n = 0;
+    for (const Cand& k : cands) {
+        c.tier = k.tier;
+        c.layout = k.layout;
+        c.chunk_cap = k.chunk;
+        uint32_t nf = 0;
+        rc = launch_common(ctx, p, rk, 0, std::max<uint32_t>(1, sample / 8), got.data(), &nf, nullptr, nullptr,
+                           nullptr, nullptr);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (rc == CK_OK) rc = launch_common(ctx, p, rk, 0, sample, got.data(), &nf, nullptr, nullptr, nullptr, nullptr);
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != CK_OK) break;
+        std::vector<uint32_t> f(got.begin(), got.begin() + nf);
+        if (!have_want) {
+            want = f;
+            have_want = true;
+        } else if (f != want) {
+            rc = fail(CK_ERANGE, "tuning candidates disagree on the failing trials");
+            break;
+        }
+        const double rate = sample / std::max(dt, 1e-9);
+        if (*n < cap) entries[*n] = ck_tune_entry{k.tier, k.layout, k.chunk, 0, rate};
+        ++*n;
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = k;
+        }
+    }
+    c.tier = rc == CK_OK ? best.tier : saved_tier;
+    c.layout = rc == CK_OK ? best.layout : saved_layout;
+    c.chunk_cap = rc == CK_OK ? best.chunk : saved_chunk;
+    return rc;
 }
 
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout) {

@@ -36,6 +36,22 @@ void worker(int id, int device, const Options& o, const ck_params& p, const std:
     check(ck_ctx_set_tier(ctx.h, tier_code(o.tier)), "ck_ctx_set_tier");
     const uint64_t batches = total_batches(o);
     const uint32_t micro = static_cast<uint32_t>(std::min<uint64_t>(o.micro, 1ULL << o.bt));
+    std::string tuned;
+    if (o.tune) {
+        // A sample of batch 0 under every placement; results are not committed.
+        const uint32_t sample = static_cast<uint32_t>(std::min<uint64_t>(batch_length(o, 0), 1u << 17));
+        const BatchKey key = batch_key(o.seed, 0);
+        ck_tune_entry ents[32];
+        uint32_t n = 0;
+        check(ck_ctx_tune(ctx.h, &p, key.data(), sample, ents, 32, &n), "ck_ctx_tune");
+        const ck_tune_entry* best = nullptr;
+        for (uint32_t i = 0; i < std::min<uint32_t>(n, 32); ++i)
+            if (!best || ents[i].trials_per_s > best->trials_per_s) best = &ents[i];
+        if (best)
+            tuned = " tuned=tier" + std::to_string(best->tier) + "/layout" + std::to_string(best->layout) + "/chunk" +
+                    std::to_string(best->chunk) + "@" + std::to_string(static_cast<uint64_t>(best->trials_per_s)) +
+                    "/s";
+    }
     std::unique_ptr<FILE, decltype(&fclose)> journal(open_journal(o, p, id), fclose);
     std::vector<uint32_t> chunk(micro);
     {
@@ -45,7 +61,8 @@ void worker(int id, int device, const Options& o, const ck_params& p, const std:
         std::cerr << "Worker " << id << " GPU " << device << " (" << prop.name << ") bins=" << p.buckets
                   << " table=" << (o.tier == "auto" ? "auto" : o.tier) << " micro=" << micro << " k=" << p.k
                   << " stash=" << p.stash << " kick=" << p.kick << " half_bits=" << p.half_bits
-                  << " CUs=" << prop.multiProcessorCount << " lds_per_block=" << prop.sharedMemPerBlock << "\n";
+                  << " CUs=" << prop.multiProcessorCount << " lds_per_block=" << prop.sharedMemPerBlock << tuned
+                  << "\n";
     }
     while (!s.stop.load(std::memory_order_relaxed)) {
         if (pause_requested(o)) {

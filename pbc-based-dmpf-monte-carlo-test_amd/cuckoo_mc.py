@@ -67,6 +67,11 @@ class EngineStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("tier", C.c_uint32), ("layout", C.c_uint32), ("chunk", C.c_uint32), ("pad", C.c_uint32),
+                ("trials_per_s", C.c_double)]
+
+
 class CkError(RuntimeError):
     pass
 
@@ -96,6 +101,7 @@ def _load():
                                       P(Stats)]),
         "ck_trial_map": (C.c_int, [vp, P(Params), P(U32), U32, P(U32), P(U64), P(U64), P(U32)]),
         "ck_last_error": (C.c_char_p, []),
+        "ck_ctx_tune": (C.c_int, [vp, P(Params), P(U32), U32, P(TuneEntry), U32, P(U32)]),
         "ck_engine_create": (C.c_int, [C.c_int, P(EngineConfig), P(vp)]),
         "ck_engine_destroy": (None, [vp]),
         "ck_engine_submit": (C.c_int, [vp, P(Job), P(U64)]),
@@ -117,7 +123,7 @@ LIB = _load()
 # Every symbol declared in include/cuckoo_mc.h.
 EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "ck_ctx_destroy",
            "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_ctx_set_layout", "ck_aes_self_test", "ck_launch",
-           "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_engine_create",
+           "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_ctx_tune", "ck_engine_create",
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
            "ck_engine_release", "ck_engine_stats_get"]
 
@@ -187,6 +193,16 @@ class Context:
 
     def self_test(self):
         _check(LIB.ck_aes_self_test(self.h))
+
+    def tune(self, p: Params, rk: np.ndarray, sample: int):
+        """Times every applicable placement on trials [0, sample) and keeps the
+        fastest; returns [{tier, layout, chunk, trials_per_s}] in run order."""
+        rk = np.ascontiguousarray(rk, dtype=np.uint32)
+        ents = (TuneEntry * 32)()
+        n = C.c_uint32()
+        _check(LIB.ck_ctx_tune(self.h, C.byref(p), _u32p(rk), sample, ents, 32, C.byref(n)))
+        return [{"tier": e.tier, "layout": e.layout, "chunk": e.chunk, "trials_per_s": e.trials_per_s}
+                for e in ents[:min(n.value, 32)]]
 
     def launch(self, p: Params, rk: np.ndarray, first: int, count: int):
         """Failing offsets (sorted) of trials [first, first+count) and stats."""

@@ -301,3 +301,32 @@ def test_residual_layout(ctx, mode, pnum, ell, kw, count):
     for i in range(0, count, max(1, count // 8)):
         _, bins, _ = ock.trial_run(bk, 5 + i, op)
         assert int(dig[i]) == ock.table_digest(bins, ock.bucket_map(bk, 5 + i, op), op.k), i
+
+
+@pytest.mark.parametrize("mode,pnum,ell,kw,sample", [
+    ("bgh25", 1024, 16, {"kick": 500}, 4096),                     # both tiers, four layouts
+    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50}, 2048),
+])
+def test_autotuner(mode, pnum, ell, kw, sample):
+    """Every candidate reports the same failures (checked inside the tuner);
+    the chosen setting keeps giving the oracle's results."""
+    p, op = both(mode, pnum, ell, **kw)
+    rk = ck.batch_key(77, 3)
+    with ck.Context(0) as c:
+        ents = c.tune(p, rk, sample)
+        assert len(ents) >= 4 and all(e["trials_per_s"] > 0 for e in ents)
+        assert {e["layout"] for e in ents if e["tier"] == ck.TIER_HBM} >= {ck.LAYOUT_MAP, ck.LAYOUT_MAP_XCHG,
+                                                                          ck.LAYOUT_RESIDUAL}
+        f, _ = c.launch(p, rk, 100, 700)
+    of, _, _ = ock.run_range(ock.batch_key(77, 3), 100, 700, op)
+    assert list(f) == list(of)
+
+
+def test_config1_cpu_point(ctx):
+    """The reference's CPU-runnable point (bgh25 n=1024 m=1536 k=3 kick 500,
+    ell_i 16, seed 0, 10^5 trials of batch 0): failures and probe total."""
+    p, op = both("bgh25", 1024, 16, kick=500)
+    f, st = ctx.launch(p, ck.batch_key(0, 0), 0, 100000)
+    of, _, ost = ock.run_range(ock.batch_key(0, 0), 0, 100000, op)
+    assert list(f) == list(of)
+    assert st.probes == ost.probes
