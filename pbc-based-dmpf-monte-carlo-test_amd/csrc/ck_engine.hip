@@ -111,9 +111,15 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build_mix(const SegDev* 
     }
 }
 
-// chain: one wave per segment, one lane per trial.  M = u64 selects the
-// descriptor layout (u64 entries), otherwise u32 map and u32 entries.
-template <int K, bool BGH, class M>
+// Table layouts of a job (its group keeps one per pass):
+//   LAY_MAP  u32 bucket-map rows, u32 entries ordinal | choice << 30;
+//   LAY_DESC u64 packed candidate stream, u64 entries candidates | choice.
+// (Residual entries, the single-shape path's third form, measured slower here:
+// their 36-38 VGPR chain waves crowd the builds that share the CUs.)
+constexpr uint32_t LAY_MAP = 0, LAY_DESC = 1;
+
+// chain: one wave per segment, one lane per trial.
+template <int K, bool BGH, uint32_t LAY>
 __global__ __launch_bounds__(256) void k_chain_mix(const SegDev* __restrict__ segs, uint32_t nsegs,
                                                    char* __restrict__ arena, const uint64_t* __restrict__ walks,
                                                    const uint64_t* __restrict__ rnds, JobCounters* __restrict__ jc,
@@ -127,8 +133,8 @@ __global__ __launch_bounds__(256) void k_chain_mix(const SegDev* __restrict__ se
     bool failed = false;
     uint32_t pr = 0;
     if (lane < G.count) {
-        const M* m = reinterpret_cast<const M*>(arena + G.map_off + size_t(lane) * G.map_bytes);
-        if constexpr (sizeof(M) == 8) {
+        if constexpr (LAY == LAY_DESC) {
+            const uint64_t* m = reinterpret_cast<const uint64_t*>(arena + G.map_off + size_t(lane) * G.map_bytes);
             uint64_t* t = reinterpret_cast<uint64_t*>(arena + G.table_off + size_t(lane) * G.table_bytes);
             const DescCodec D{S.dbits, K * S.dbits, (1ull << S.dbits) - 1, (1ull << (K * S.dbits)) - 1};
             if constexpr (BGH)
@@ -136,6 +142,7 @@ __global__ __launch_bounds__(256) void k_chain_mix(const SegDev* __restrict__ se
             else
                 failed = chain_drrt_desc<K>(S, D, m, t, pr);
         } else {
+            const uint32_t* m = reinterpret_cast<const uint32_t*>(arena + G.map_off + size_t(lane) * G.map_bytes);
             uint32_t* t = reinterpret_cast<uint32_t*>(arena + G.table_off + size_t(lane) * G.table_bytes);
             if constexpr (BGH)
                 failed = chain_bgh<K>(S, m, t, walks[slot], rnds[slot], pr);
@@ -175,7 +182,8 @@ struct Job {
     uint64_t id = 0;
     ck_job spec{};
     Shape S{};
-    bool cached = false, desc = false, bgh = true;
+    bool cached = false, bgh = true;
+    uint32_t lay = LAY_MAP;
     uint32_t group = 0;
     size_t map_bytes = 0, table_bytes = 0;  // per slot
     uint64_t scheduled = 0, completed = 0;
@@ -260,7 +268,7 @@ static int set_fail(Engine& E, const std::string& msg) {
 }
 
 static uint32_t group_of(const Job& j) {
-    return (j.spec.p.k << 3) | (j.bgh ? 4u : 0u) | (j.desc ? 2u : 0u) | (j.cached ? 1u : 0u);
+    return (j.spec.p.k << 4) | (j.bgh ? 8u : 0u) | (j.lay << 1) | (j.cached ? 1u : 0u);
 }
 
 // Slots of job j that fit in `slots_left` slots and `arena_left` bytes,
@@ -423,19 +431,19 @@ static void launch_build(Engine& E, const PassBuf& B, bool cached, bool desc, ui
 }
 
 template <int K>
-static void launch_chain(Engine& E, const PassBuf& B, bool bgh, bool desc) {
+static void launch_chain(Engine& E, const PassBuf& B, bool bgh, uint32_t lay) {
     const dim3 grid((B.nsegs + 3) / 4), block(256);
-#define CH(BG, M)                                                                                                \
-    hipLaunchKernelGGL((k_chain_mix<K, BG, M>), grid, block, 0, E.cstream[B.cs], B.segs_d, B.nsegs, B.arena, B.walks, \
+#define CH(BG, L)                                                                                                \
+    hipLaunchKernelGGL((k_chain_mix<K, BG, L>), grid, block, 0, E.cstream[B.cs], B.segs_d, B.nsegs, B.arena, B.walks, \
                        B.rnds, E.jc_d, B.pc_d, B.fail_d)
-    if (bgh && desc)
-        CH(true, uint64_t);
+    if (bgh && lay == LAY_DESC)
+        CH(true, LAY_DESC);
     else if (bgh)
-        CH(true, uint32_t);
-    else if (desc)
-        CH(false, uint64_t);
+        CH(true, LAY_MAP);
+    else if (lay == LAY_DESC)
+        CH(false, LAY_DESC);
     else
-        CH(false, uint32_t);
+        CH(false, LAY_MAP);
 #undef CH
 }
 
@@ -447,9 +455,9 @@ static int enqueue_pass(Engine& E, int bi) {
     EH(hipMemcpyAsync(B.segs_d, B.segs_h, size_t(B.nsegs) * sizeof(SegDev), hipMemcpyHostToDevice, E.bstream));
     EH(hipEventRecord(B.eb0, E.bstream));
     switch (lead.spec.p.k) {
-        case 2: launch_build<2>(E, B, lead.cached, lead.desc, max_half); break;
-        case 3: launch_build<3>(E, B, lead.cached, lead.desc, max_half); break;
-        default: launch_build<4>(E, B, lead.cached, lead.desc, max_half); break;
+        case 2: launch_build<2>(E, B, lead.cached, lead.lay == LAY_DESC, max_half); break;
+        case 3: launch_build<3>(E, B, lead.cached, lead.lay == LAY_DESC, max_half); break;
+        default: launch_build<4>(E, B, lead.cached, lead.lay == LAY_DESC, max_half); break;
     }
     EH(hipGetLastError());
     EH(hipEventRecord(B.eb1, E.bstream));
@@ -459,9 +467,9 @@ static int enqueue_pass(Engine& E, int bi) {
     EH(hipMemsetAsync(B.pc_d, 0, PC_WORDS * 4, cst));
     EH(hipEventRecord(B.ec0, cst));
     switch (lead.spec.p.k) {
-        case 2: launch_chain<2>(E, B, lead.bgh, lead.desc); break;
-        case 3: launch_chain<3>(E, B, lead.bgh, lead.desc); break;
-        default: launch_chain<4>(E, B, lead.bgh, lead.desc); break;
+        case 2: launch_chain<2>(E, B, lead.bgh, lead.lay); break;
+        case 3: launch_chain<3>(E, B, lead.bgh, lead.lay); break;
+        default: launch_chain<4>(E, B, lead.bgh, lead.lay); break;
     }
     EH(hipGetLastError());
     EH(hipEventRecord(B.ec1, cst));
@@ -743,11 +751,11 @@ int ck_engine_submit(ck_engine* h, const ck_job* job, uint64_t* job_id) {
     j->S = make_shape(job->p);
     j->cached = job->p.half_bits <= kMaxCacheHalf &&
                 tab_bytes(kRepl) + (size_t(8) << job->p.half_bits) * 2 <= E.lds_block;
-    j->desc = job->p.k * j->S.dbits + 2 <= 63;
+    j->lay = job->p.k * j->S.dbits + 2 <= 63 ? LAY_DESC : LAY_MAP;
     j->bgh = job->p.mode == CK_MODE_BGH25;
     j->group = group_of(*j);
-    j->map_bytes = align256(size_t(job->p.pnum) * (j->desc ? 8 : size_t(job->p.k) * 4));
-    j->table_bytes = align256(size_t(job->p.buckets) * (j->desc ? 8 : 4));
+    j->map_bytes = align256(size_t(job->p.pnum) * (j->lay == LAY_DESC ? 8 : size_t(job->p.k) * 4));
+    j->table_bytes = align256(size_t(job->p.buckets) * (j->lay == LAY_DESC ? 8 : 4));
     if ((j->map_bytes + j->table_bytes) * SEG + 1024 > E.arena_per)
         return fail(CK_ENOMEM, "64 trials of this shape do not fit one pass buffer");
     j->bw = build_weight(job->p, j->cached);

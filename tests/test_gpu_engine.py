@@ -25,7 +25,8 @@ JOBS = [
     ("bgh25", 1000, 16, dict(k=4, buckets=1200, kick=1000, stash=4), 15, 321, 20),
     ("bgh25", 16, 32, dict(), 16, 200, 20),                          # half 17: direct-AES group
     ("bgh25", 3000, 16, dict(buckets=3900, kick=100), 17, 500, 6),    # 64-trial batches
-    ("drrt18", 30000, 16, dict(k=4, buckets=40000, kick=100), 18, 70, 20),  # k*bits > 61: map layout
+    ("drrt18", 30000, 16, dict(k=4, buckets=40000, kick=100), 18, 70, 20),  # 4*16 + 2 > 63: bucket-map group
+    ("bgh25", 40000, 16, dict(k=4, buckets=50000, kick=200, stash=1), 21, 40, 20),
     ("drrt18", 16, 8, dict(), 19, 999, 8),
     ("bgh25", 64, 10, dict(k=3, buckets=70, kick=30, stash=3, input_mode=1), 20, 1500, 9),
 ]
@@ -173,3 +174,16 @@ def test_sweep_points_run_bit_exact():
         of, op = oracle_job(pt["mode"], pt["pnum"], pt["ell_i"], kw, pt["seed"], pt["trials"], pt["bt"])
         assert fails == of, pt
         assert r.probes == op, pt
+
+
+def test_engine_bucket_map_group():
+    """n = 2^20, k = 4 with 2^21 buckets: the bucket-map group at its largest
+    entries (21-bit bucket indices, 16 MB of map per trial)."""
+    job = ("bgh25", 1 << 20, 21, dict(k=4, buckets=1 << 21, kick=100, stash=1), 23, 12, 20)
+    ofails, oprobes = oracle_job(*job)
+    m, n, ell, kw, seed, trials, bt = job
+    with ck.Engine(0, arena_bytes=8 * GiB, buffers=2, max_slots=1024) as eng:
+        jid = eng.submit(ck.params(m, n, ell, **kw), seed, trials, bt=bt, keep_failures=True)
+        r = eng.wait(jid)
+        assert list(map(int, eng.failures(jid))) == ofails
+        assert r.probes == oprobes
