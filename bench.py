@@ -128,6 +128,27 @@ def cpu_reference(cores: int, gpu_failures_prefix):
             "gpu_failures_equal": failed == gpu_failures_prefix}
 
 
+def random_access_ceiling(probes_per_s: float):
+    """Runs lib/ck_randbench: the chain's access pattern (65536 lanes, each a
+    dependent random read + write of one 8-byte slot of its own 192 KB table)
+    without the chain's logic.  Its rate is the achievable ceiling for the
+    insertion chains on this GPU."""
+    import re
+    import subprocess
+    from paths import PKG
+    exe = PKG / "lib" / "ck_randbench"
+    if not exe.exists():
+        return None
+    out = subprocess.run([str(exe), "65536", "196608", "32768"], capture_output=True, text=True, timeout=300)
+    rates = dict(re.findall(r"^(read_write|read_only) .*steps_per_s=([0-9.e+]+)", out.stdout, re.M))
+    if out.returncode != 0 or "read_write" not in rates:
+        return {"error": out.stderr[-300:]}
+    rw, ro = float(rates["read_write"]), float(rates.get("read_only", 0))
+    return {"pattern": "65536 lanes x 192 KB tables, dependent random 8-byte read + write per step",
+            "read_write_steps_per_s": rw, "read_only_steps_per_s": ro,
+            "chain_probes_per_s": probes_per_s, "frac": probes_per_s / rw}
+
+
 def load_sweep():
     import importlib.util
     from paths import PKG
@@ -440,6 +461,7 @@ def main():
         },
         "device_ms": {"build": agg["build_ms"], "chain": agg["chain_ms"]},
     }
+    rec["roofline"]["random_access_ceiling"] = random_access_ceiling(agg["probes"] / chain_launches / chain_avg_s)
     ok = True
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)
