@@ -36,7 +36,7 @@ __device__ __forceinline__ LdsTab stage_tab_from(const uint32_t* src, uint32_t* 
     const uint32_t n = 256u << shift;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = src[i >> shift];
     __syncthreads();
-    return LdsTab{lds, threadIdx.x & ((1u << shift) - 1u), shift};
+    return make_tab(lds, shift);
 }
 
 __device__ __forceinline__ void derive_uniform(const LdsTab& T, const RoundKeys& bk, const Shape& S, uint32_t offset,

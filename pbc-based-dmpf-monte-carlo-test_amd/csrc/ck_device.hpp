@@ -33,17 +33,31 @@ struct RoundKeys {
 };
 
 __device__ __forceinline__ uint32_t rotr(uint32_t x, uint32_t s) { return __builtin_amdgcn_alignbit(x, x, s); }
+// a ^ b ^ c in one VALU op (v_bitop3_b32, truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 
 // T0 in LDS, replicated 2^shift times: entry x of copy c sits at
 // (x << shift) | c.  With shift 5 a lane L reading copy L%32 touches bank L%32
 // whatever the byte index (no bank conflicts); shift 0 is the plain 1 KB table.
+// Lookups form the LDS byte address (x << (shift + 2)) + (base + copy * 4)
+// with one shift-add and read through an LDS-space pointer, so a lookup costs
+// a byte extract, that shift-add and the read.
+typedef const __attribute__((address_space(3))) uint32_t* lds_u32_cptr;
 struct LdsTab {
-    const uint32_t* tab;
-    uint32_t lane;   // lane % 2^shift
-    uint32_t shift;
-    __device__ __forceinline__ uint32_t t(uint32_t x) const { return tab[(x << shift) | lane]; }
+    uint32_t lane4b;  // LDS byte address of this lane's copy of entry 0
+    uint32_t shift2;  // shift + 2
+    __device__ __forceinline__ uint32_t t(uint32_t x) const {
+        return *reinterpret_cast<lds_u32_cptr>(static_cast<uintptr_t>((x << shift2) + lane4b));
+    }
     __device__ __forceinline__ uint32_t s(uint32_t x) const { return (t(x) >> 16) & 255u; }
 };
+__device__ __forceinline__ LdsTab make_tab(const uint32_t* lds, uint32_t shift) {
+    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_u32_cptr)lds));
+    return LdsTab{base + ((threadIdx.x & ((1u << shift) - 1u)) << 2), shift + 2};
+}
 
 // FIPS-197 AES-128 encryption in the T-table form (Daemen & Rijmen): rounds
 // 1..9 are T0[b0]^ror8(T0[b1])^ror16(T0[b2])^ror24(T0[b3])^rk, the last round
@@ -53,14 +67,14 @@ __device__ __forceinline__ void aes_enc(const LdsTab& T, const uint32_t* rk, uin
     uint32_t s0 = a ^ rk[0], s1 = b ^ rk[1], s2 = c ^ rk[2], s3 = d ^ rk[3];
 #pragma unroll
     for (int r = 1; r < 10; ++r) {
-        const uint32_t n0 = T.t(s0 >> 24) ^ rotr(T.t((s1 >> 16) & 255), 8) ^ rotr(T.t((s2 >> 8) & 255), 16) ^
-                            rotr(T.t(s3 & 255), 24) ^ rk[4 * r];
-        const uint32_t n1 = T.t(s1 >> 24) ^ rotr(T.t((s2 >> 16) & 255), 8) ^ rotr(T.t((s3 >> 8) & 255), 16) ^
-                            rotr(T.t(s0 & 255), 24) ^ rk[4 * r + 1];
-        const uint32_t n2 = T.t(s2 >> 24) ^ rotr(T.t((s3 >> 16) & 255), 8) ^ rotr(T.t((s0 >> 8) & 255), 16) ^
-                            rotr(T.t(s1 & 255), 24) ^ rk[4 * r + 2];
-        const uint32_t n3 = T.t(s3 >> 24) ^ rotr(T.t((s0 >> 16) & 255), 8) ^ rotr(T.t((s1 >> 8) & 255), 16) ^
-                            rotr(T.t(s2 & 255), 24) ^ rk[4 * r + 3];
+        const uint32_t n0 = xor3(xor3(T.t(s0 >> 24), rotr(T.t((s1 >> 16) & 255), 8), rotr(T.t((s2 >> 8) & 255), 16)),
+                                 rotr(T.t(s3 & 255), 24), rk[4 * r]);
+        const uint32_t n1 = xor3(xor3(T.t(s1 >> 24), rotr(T.t((s2 >> 16) & 255), 8), rotr(T.t((s3 >> 8) & 255), 16)),
+                                 rotr(T.t(s0 & 255), 24), rk[4 * r + 1]);
+        const uint32_t n2 = xor3(xor3(T.t(s2 >> 24), rotr(T.t((s3 >> 16) & 255), 8), rotr(T.t((s0 >> 8) & 255), 16)),
+                                 rotr(T.t(s1 & 255), 24), rk[4 * r + 2]);
+        const uint32_t n3 = xor3(xor3(T.t(s3 >> 24), rotr(T.t((s0 >> 16) & 255), 8), rotr(T.t((s1 >> 8) & 255), 16)),
+                                 rotr(T.t(s2 & 255), 24), rk[4 * r + 3]);
         s0 = n0;
         s1 = n1;
         s2 = n2;
@@ -182,9 +196,9 @@ __device__ __forceinline__ void aes_last_n(const LdsTab& T, const uint32_t* rk, 
         for (int i = 0; i < N; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                n[i][j] = T.t(s[i][j] >> 24) ^ rotr(T.t((s[i][(j + 1) & 3] >> 16) & 255), 8) ^
-                          rotr(T.t((s[i][(j + 2) & 3] >> 8) & 255), 16) ^ rotr(T.t(s[i][(j + 3) & 3] & 255), 24) ^
-                          rk[4 * r + j];
+                n[i][j] = xor3(xor3(T.t(s[i][j] >> 24), rotr(T.t((s[i][(j + 1) & 3] >> 16) & 255), 8),
+                                    rotr(T.t((s[i][(j + 2) & 3] >> 8) & 255), 16)),
+                               rotr(T.t(s[i][(j + 3) & 3] & 255), 24), rk[4 * r + j]);
 #pragma unroll
         for (int i = 0; i < N; ++i)
 #pragma unroll

@@ -42,7 +42,7 @@ __device__ __forceinline__ LdsTab stage_tab(uint32_t* lds, uint32_t shift) {
     const uint32_t n = 256u << shift;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = c_te0[i >> shift];
     __syncthreads();
-    return LdsTab{lds, threadIdx.x & ((1u << shift) - 1u), shift};
+    return make_tab(lds, shift);
 }
 
 // build (HBM tier): one wave per trial slot.  CACHED selects the round-cache
