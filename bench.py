@@ -301,6 +301,36 @@ def config1_leg(ck):
             "failures_equal": list(f) == list(of)}, list(f) == list(of)
 
 
+def graph_leg(ck, p, steps: int, trials: int):
+    """Graph mode against kernel-by-kernel launches on three launch sizes:
+    the headline steps (2^20 trials), the n = 1024 point (1e5 trials per
+    launch) and small n = 256 launches of 1,024 trials, where host launch work
+    weighs most.  Each form runs in its own context after one untimed launch
+    (which builds the graph); the failure lists of every launch must agree."""
+    cases = [("headline n=16384, 2^20 trials per launch", p, trials, steps),
+             ("n=1024 m=1536 kick 500, 1e5 trials per launch", ck.params("bgh25", 1024, 16, kick=500), 100000, 20),
+             ("n=256 m=384 kick 500, 1024 trials per launch", ck.params("bgh25", 256, 16, kick=500), 1024, 200)]
+    out = []
+    for name, shp, count, reps in cases:
+        rate, fails = {}, {}
+        for mode in ("kernels", "graph"):
+            with ck.Context(0) as ctx:
+                ctx.set_graph(mode == "graph")
+                ctx.launch(shp, ck.batch_key(SEED, 0), 0, count)
+                got = []
+                t0 = time.perf_counter()
+                for r in range(reps):
+                    f, st = ctx.launch(shp, ck.batch_key(SEED, 1 + r), 0, count)
+                    got.append(f)
+                rate[mode] = reps * count / (time.perf_counter() - t0)
+                fails[mode] = [list(map(int, f)) for f in got]
+        out.append({"workload": name, "launches": reps, "kernel_by_kernel_trials_per_s": rate["kernels"],
+                    "graph_trials_per_s": rate["graph"], "graph_vs_kernels": rate["graph"] / rate["kernels"],
+                    "results_equal": fails["kernels"] == fails["graph"]})
+        print(f"graph: {name}: {rate['graph']:.0f} vs {rate['kernels']:.0f} trials/s", file=sys.stderr, flush=True)
+    return out
+
+
 def hbm_leg(ck, trials: int, layout: int = 0):
     """Rare-event HBM shape: n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
     ell_i 20, seed 4 -- the first `trials` trials of batch 0."""
@@ -313,10 +343,15 @@ def hbm_leg(ck, trials: int, layout: int = 0):
     dt = time.perf_counter() - t0
     ctx.close()
     chain_s = st.chain_ms / 1e3
+    passes = max(1, st.launches // 2)
+    pass_s = (st.build_ms + st.chain_ms) / 1e3 / passes  # single-buffered: a pass is its build, then its chain
     return {
         "workload": f"bgh25 n=2^20 m=1363149 k=3 stash=2 kick=1000 ell_i=20 seed=4, trials [0, {trials}) of batch 0",
         "trials_per_s": trials / dt, "probes_per_s": st.probes / dt, "probes_per_trial": st.probes / trials,
-        "failed": len(f), "trials_per_pass": st.chunk,
+        "failed": len(f), "trials_per_pass": st.chunk, "passes": passes,
+        # Chains are latency-bound, so a pass takes about as long whatever its trial
+        # count: a long run of full passes goes at trials_per_pass / pass time.
+        "full_pass_trials_per_s": st.chunk / pass_s if pass_s > 0 else None,
         "chain_algorithmic_gbs": SLOT_BYTES * st.probes / chain_s / 1e9 if chain_s > 0 else None,
         "device_ms": {"build": st.build_ms, "chain": st.chain_ms},
     }
@@ -347,6 +382,7 @@ def main():
                     help="trials of the n=2^20 leg (default: two full memory-capped passes)")
     ap.add_argument("--no-hbm", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-mode comparison")
     ap.add_argument("--tune-sample", type=int, default=1 << 17, help="trials per autotuner candidate")
     ap.add_argument("--hbm-only", action="store_true", help="run only the n=2^20 leg (for counter profiles)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -476,6 +512,9 @@ def main():
         best = max(ents, key=lambda e: e["trials_per_s"])
         print(f"autotune: best tier {best['tier']} layout {names[best['layout']]} chunk {best['chunk']} "
               f"{best['trials_per_s']:.0f} trials/s", file=sys.stderr, flush=True)
+    if not args.no_graph:
+        rec["graph"] = graph_leg(ck, p, args.steps, t)
+        ok = ok and all(g["results_equal"] for g in rec["graph"])
     if not args.no_cpu:
         rec["config1"], good = config1_leg(ck)
         ok = ok and good

@@ -73,7 +73,7 @@ typedef struct ck_stats {
     double chain_ms;      /* device time of the insertion kernels */
     double fused_ms;      /* device time of fused (LDS tier) kernels */
     uint32_t launches;    /* kernel launches */
-    uint32_t pad;
+    uint32_t graph;       /* 0: launched kernel by kernel; 1: replayed this context's graph; 2: built and ran a new one */
 } ck_stats;
 
 typedef struct ck_ctx ck_ctx;
@@ -100,6 +100,17 @@ int ck_ctx_set_tier(ck_ctx* ctx, uint32_t tier);
 int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
 /* Forces an HBM-tier table layout (CK_LAYOUT_*). */
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
+/* Graph mode (on = 1): HBM-tier ck_launch calls run as one hipGraph of all
+ * their passes.  A graph is built on the first call for a shape, trial count
+ * and placement, and replayed by later calls with only the batch key and
+ * offsets rewritten -- the reference's per-micro-range launch of a fixed shape
+ * (cuckoo_bench.cu:980-1031) with the host launch work of every pass taken out.
+ * A context keeps its four most recently used graphs.
+ * Results are those of the kernel-by-kernel form.  Launches that ask for
+ * per-trial outputs (ck_launch_debug, ck_launch_digest) stay kernel by kernel.
+ * With graph mode, stats->chain_ms is the device time of the whole replay and
+ * stats->build_ms is 0. */
+int ck_ctx_set_graph(ck_ctx* ctx, uint32_t on);
 
 /* One autotuner measurement. */
 typedef struct ck_tune_entry {

@@ -91,7 +91,7 @@ static const char* kUsage =
     "       [--prp-cache auto|on|off] [--block-threads 256] [--table-layout auto|global|shared]\n"
     "       [--shared-variant auto|round|map] [--warps-per-block N] [--stop-after 1048576]\n"
     "       [--global-table] [--status]\n"
-    "  extension: [--k 2..4] [--buckets M] [--kick N] [--stash S] [--trials N] [--tier auto|hbm|lds] [--tune]\n";
+    "  extension: [--k 2..4] [--buckets M] [--kick N] [--stash S] [--trials N] [--tier auto|hbm|lds] [--tune] [--graph]\n";
 
 // Flags and validation of cuckoo_bench.cu:431-480; layout flags are accepted
 // for compatibility (the tier is chosen by --tier / the library).
@@ -113,6 +113,10 @@ Options parse(int argc, char** argv) {
         }
         if (a == "--tune") {
             o.tune = true;
+            continue;
+        }
+        if (a == "--graph") {
+            o.graph = true;
             continue;
         }
         if (i + 1 >= argc) throw std::runtime_error("Missing value after " + a);

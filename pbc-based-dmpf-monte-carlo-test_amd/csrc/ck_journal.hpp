@@ -46,7 +46,8 @@ struct Options {
     uint32_t k = 3, buckets = CK_DERIVED, kick = CK_DERIVED, stash = 0;
     uint64_t trials = 0;  // 0: planned = 2^et
     std::string tier = "auto";
-    bool tune = false;  // --tune: each worker times the placements on a sample first
+    bool tune = false;   // --tune: each worker times the placements on a sample first
+    bool graph = false;  // --graph: each worker's launches replay a hipGraph per micro-range size
 };
 
 Options parse(int argc, char** argv);

@@ -20,6 +20,8 @@
 #include <chrono>
 #include <cstring>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "ck_build.hpp"
@@ -335,6 +337,73 @@ struct Workspace {
     size_t bytes = 0;
 };
 
+// Where a kernel goes: onto a stream, into a graph as a new node after `deps`,
+// or into an existing node of an instantiated graph (new argument values; the
+// kernel and its launch shape stay).  All three pass the arguments as one
+// array of pointers to values of the kernel's own parameter types.
+struct Dispatch {
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    const hipGraphNode_t* deps = nullptr;
+    size_t ndeps = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t node = nullptr;  // added node (graph), or the node to rewrite (exec)
+    hipError_t err = hipSuccess;
+
+    template <class... P, class... A>
+    void operator()(void (*k)(P...), dim3 grid, dim3 block, size_t lds, A&&... args) {
+        std::tuple<std::decay_t<P>...> vals(std::forward<A>(args)...);
+        put(reinterpret_cast<void*>(k), grid, block, lds, vals, std::index_sequence_for<P...>{});
+    }
+
+   private:
+    template <class T, size_t... I>
+    void put(void* fn, dim3 grid, dim3 block, size_t lds, T& vals, std::index_sequence<I...>) {
+        void* ptrs[] = {static_cast<void*>(&std::get<I>(vals))...};
+        hipError_t e;
+        if (!graph && !exec) {
+            e = hipLaunchKernel(fn, grid, block, ptrs, lds, stream);
+        } else {
+            hipKernelNodeParams kp{};
+            kp.func = fn;
+            kp.gridDim = grid;
+            kp.blockDim = block;
+            kp.sharedMemBytes = static_cast<unsigned>(lds);
+            kp.kernelParams = ptrs;
+            kp.extra = nullptr;
+            e = graph ? hipGraphAddKernelNode(&node, graph, deps, ndeps, &kp)
+                      : hipGraphExecKernelNodeSetParams(exec, node, &kp);
+        }
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    }
+};
+
+// An HBM-tier launch as a graph (graph mode): every pass's build, counter
+// clear and chain as graph nodes with the stream order's dependencies,
+// replayed for the same shape and trial count with new batch keys and
+// offsets.  Each pass has its own counters and failure list in `aux`, so
+// nothing is read back until the whole graph has run.
+struct HbmGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ck_params p{};
+    uint32_t count = 0, layout = 0, chunk = 0, nbuf = 0;
+    void* ws = nullptr;
+    uint64_t last_use = 0;
+    std::vector<hipGraphNode_t> build, chain;
+    uint32_t* aux = nullptr;  // per pass: CNT_WORDS counters, then up to `chunk` failure offsets
+    size_t aux_words = 0;
+    void drop() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (g) (void)hipGraphDestroy(g);
+        exec = nullptr;
+        g = nullptr;
+        build.clear();
+        chain.clear();
+        ws = nullptr;
+    }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;        // chains (HBM tier) and fused kernels (LDS tier)
@@ -350,6 +419,10 @@ struct Ctx {
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
     uint32_t cus = 1;
+    bool graph_on = false;  // HBM-tier launches run as replayed graphs
+    static constexpr int kGraphs = 4;  // graphs kept per context, least recently used replaced first
+    HbmGraph graphs[kGraphs];
+    uint64_t graph_clock = 0;
 };
 
 // Round caches above 2^12 entries per round do not pay for themselves in LDS.
@@ -509,7 +582,7 @@ static size_t slot_bytes(const ck_params& p, uint32_t layout) {
 }
 // Pass buffers: two, so the build of pass i+1 runs under the chain of pass i,
 // unless memory caps a pass below its target.  Then one buffer of twice the
-// slots: a memory-capped shape (n = 2^20: 17.5 MB per slot) is bound by the
+// slots: a memory-capped shape (n = 2^20: 19.3 MB per slot) is bound by the
 // latency of its chains, and its build is a few percent of the chain time.
 static uint32_t hbm_buffers(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout) {
     const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
@@ -521,15 +594,15 @@ static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint
     return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>({fit, size_t(target), size_t(count)})));
 }
 
-// HBM budget of the two pass buffers: most of the free device memory (the
-// current workspace counts as free, since growing it frees it first), at most
-// 240 GiB.  Large shapes (n = 2^20: 17.5 MB per trial slot) are latency-bound,
-// so every extra slot is an extra chain in flight.
-static constexpr size_t kBudgetCap = size_t(240) << 30;
+// HBM budget of the pass buffers: 95 % of the free device memory (the current
+// workspace counts as free, since growing it frees it first), at most 272 GiB
+// of the 288.  Large shapes (n = 2^20: 19.3 MB per trial slot) are
+// latency-bound, so every extra slot is an extra chain in flight.
+static constexpr size_t kBudgetCap = size_t(272) << 30;
 static size_t hbm_budget(const Ctx& c) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return size_t(96) << 30;
-    return std::min(kBudgetCap, (free_b + c.ws.bytes) / 20 * 17);
+    return std::min(kBudgetCap, (free_b + c.ws.bytes) / 20 * 19);
 }
 
 // LDS-tier per-launch scratch: [counters | probes C | fail C | digests C | tables C x buckets].
@@ -586,18 +659,18 @@ struct PassArgs {
 };
 
 template <int K, class M>
-static void launch_build_t(const BuildPlan& bp, const PassArgs& a, hipStream_t st) {
+static void launch_build_t(const BuildPlan& bp, const PassArgs& a, Dispatch& st) {
     const uint32_t blocks = (a.count + bp.waves - 1) / bp.waves;
     M* map = reinterpret_cast<M*>(a.map);
     if (bp.cached)
-        hipLaunchKernelGGL((k_build<K, true, M>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, a.S, a.bk, a.first,
-                           a.count, kRepl, map, a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
+        st(&k_build<K, true, M>, dim3(blocks), dim3(64 * bp.waves), bp.lds, a.S, a.bk, a.first, a.count, kRepl, map,
+           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
     else
-        hipLaunchKernelGGL((k_build<K, false, M>), dim3(blocks), dim3(64 * bp.waves), bp.lds, st, a.S, a.bk, a.first,
-                           a.count, kRepl, map, a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
+        st(&k_build<K, false, M>, dim3(blocks), dim3(64 * bp.waves), bp.lds, a.S, a.bk, a.first, a.count, kRepl, map,
+           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
 }
 template <int K>
-static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (h.desc || h.stream64)
         launch_build_t<K, uint64_t>(bp, a, st);
     else if (h.map16)
@@ -607,25 +680,25 @@ static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs
 }
 
 template <int K, bool BGH, class M, class E, bool XCHG>
-static void launch_chain_t(const PassArgs& a, hipStream_t st) {
+static void launch_chain_t(const PassArgs& a, Dispatch& st) {
     const uint32_t blocks = (a.count + 255) / 256;
-    hipLaunchKernelGGL((k_chain<K, BGH, M, E, XCHG>), dim3(blocks), dim3(256), 0, st, a.S, a.first, a.count,
-                       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds,
-                       a.cnt, a.fail, a.probes);
+    st(&k_chain<K, BGH, M, E, XCHG>, dim3(blocks), dim3(256), 0, a.S, a.first, a.count,
+       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.cnt, a.fail,
+       a.probes);
 }
 template <int K, class M, class E>
-static void launch_digest_t(const PassArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_digest<K, M, E>), dim3((a.count + 255) / 256), dim3(256), 0, st, a.S, a.count,
-                       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.digests);
+static void launch_digest_t(const PassArgs& a, Dispatch& st) {
+    st(&k_digest<K, M, E>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, reinterpret_cast<const M*>(a.map),
+       a.map_stride, a.tables, a.table_bytes, a.digests);
 }
 template <int K>
-static void launch_digest_k(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+static void launch_digest_k(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (h.res && h.e32)
-        hipLaunchKernelGGL((k_digest_res<K, uint32_t>), dim3((a.count + 255) / 256), dim3(256), 0, st, a.S, a.count,
-                           a.tables, a.table_bytes, a.digests);
+        st(&k_digest_res<K, uint32_t>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
+           a.table_bytes, a.digests);
     else if (h.res)
-        hipLaunchKernelGGL((k_digest_res<K, uint64_t>), dim3((a.count + 255) / 256), dim3(256), 0, st, a.S, a.count,
-                           a.tables, a.table_bytes, a.digests);
+        st(&k_digest_res<K, uint64_t>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
+           a.table_bytes, a.digests);
     else if (h.desc)
         launch_digest_t<K, uint64_t, uint64_t>(a, st);
     else if (h.map16 && h.table16)
@@ -639,14 +712,14 @@ static void launch_digest_k(const HbmPlan& h, const PassArgs& a, hipStream_t st)
 }
 
 template <int K, bool BGH, class M, class E>
-static void launch_chain_res_t(const PassArgs& a, hipStream_t st) {
+static void launch_chain_res_t(const PassArgs& a, Dispatch& st) {
     const uint32_t blocks = (a.count + 255) / 256;
-    hipLaunchKernelGGL((k_chain_res<K, BGH, M, E>), dim3(blocks), dim3(256), 0, st, a.S, a.first, a.count,
-                       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds,
-                       a.cnt, a.fail, a.probes);
+    st(&k_chain_res<K, BGH, M, E>, dim3(blocks), dim3(256), 0, a.S, a.first, a.count,
+       reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.cnt, a.fail,
+       a.probes);
 }
 template <int K, bool BGH>
-static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (h.res && h.stream64 && h.e32)
         launch_chain_res_t<K, BGH, uint64_t, uint32_t>(a, st);
     else if (h.res && h.stream64)
@@ -673,11 +746,25 @@ static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, hipStream_t st)
         launch_chain_t<K, BGH, uint32_t, uint32_t, false>(a, st);
 }
 template <int K>
-static void launch_chain_k(bool bgh, const HbmPlan& h, const PassArgs& a, hipStream_t st) {
+static void launch_chain_k(bool bgh, const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (bgh)
         launch_chain_kb<K, true>(h, a, st);
     else
         launch_chain_kb<K, false>(h, a, st);
+}
+static void launch_build(uint32_t k, const BuildPlan& bp, const HbmPlan& h, const PassArgs& a, Dispatch& st) {
+    switch (k) {
+        case 2: launch_build_k<2>(bp, h, a, st); break;
+        case 3: launch_build_k<3>(bp, h, a, st); break;
+        default: launch_build_k<4>(bp, h, a, st); break;
+    }
+}
+static void launch_chain(uint32_t k, bool bgh, const HbmPlan& h, const PassArgs& a, Dispatch& st) {
+    switch (k) {
+        case 2: launch_chain_k<2>(bgh, h, a, st); break;
+        case 3: launch_chain_k<3>(bgh, h, a, st); break;
+        default: launch_chain_k<4>(bgh, h, a, st); break;
+    }
 }
 
 template <int K, bool BGH>
@@ -779,6 +866,130 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
     return CK_OK;
 }
 
+// Graph mode of the HBM tier.  The first call for a (shape, trial count,
+// layout, pass size, workspace) builds a graph of every pass -- build, counter
+// clear and chain nodes, with the dependencies the two streams give in the
+// eager form (builds in order; chains in order; a chain after its build; a
+// build after the chain that last used its buffer) -- and later calls rewrite
+// only the batch key and offsets of the build and chain nodes before replaying
+// it.  Every pass writes its own counters and failure list, read back once the
+// whole graph has run.  Device time is reported as chain_ms for the whole
+// replay (build_ms stays 0: the kernels of a graph are not timed one by one).
+static int run_hbm_graph(Ctx& c, const ck_params& p, uint32_t first, uint32_t count, uint32_t layout, uint32_t nbuf,
+                         uint32_t chunk, const HbmPlan& h, const BuildPlan& bp, PassArgs (&args)[2], HostOut& out,
+                         ck_stats* st) {
+    const uint32_t passes = (count + chunk - 1) / chunk;
+    const size_t stride = (CNT_WORDS + size_t(chunk) + 63) & ~size_t(63);  // u32 words per pass, 256-byte aligned
+    const bool bgh = p.mode == CK_MODE_BGH25;
+    HbmGraph* hit = nullptr;
+    HbmGraph* lru = &c.graphs[0];
+    for (HbmGraph& x : c.graphs) {
+        if (x.exec && x.ws == c.ws.base && x.count == count && x.layout == layout && x.chunk == chunk &&
+            x.nbuf == nbuf && std::memcmp(&x.p, &p, sizeof p) == 0)
+            hit = &x;
+        if (x.last_use < lru->last_use) lru = &x;
+    }
+    const bool same = hit != nullptr;
+    HbmGraph& G = same ? *hit : *lru;
+    G.last_use = ++c.graph_clock;
+    if (!same) {
+        G.drop();
+        if (G.aux_words < stride * passes) {
+            if (G.aux) CKH(hipFree(G.aux));
+            G.aux = nullptr;
+            G.aux_words = 0;
+            CKH(hipMalloc(&G.aux, stride * passes * 4));
+            G.aux_words = stride * passes;
+        }
+    }
+    // Pass i's arguments: its buffer's regions, its trial range, its counters.
+    auto pass_args = [&](uint32_t i) {
+        PassArgs a = args[nbuf == 2 ? (i & 1) : 0];
+        a.first = first + i * chunk;
+        a.count = std::min(chunk, count - i * chunk);
+        a.cnt = G.aux + i * stride;
+        a.fail = a.cnt + CNT_WORDS;
+        return a;
+    };
+    if (!same) {
+        CKH(hipGraphCreate(&G.g, 0));
+        G.build.assign(passes, nullptr);
+        G.chain.assign(passes, nullptr);
+        for (uint32_t i = 0; i < passes; ++i) {
+            const PassArgs a = pass_args(i);
+            std::vector<hipGraphNode_t> deps;
+            if (i) deps.push_back(G.build[i - 1]);
+            if (i >= nbuf) deps.push_back(G.chain[i - nbuf]);
+            Dispatch bd;
+            bd.graph = G.g;
+            bd.deps = deps.data();
+            bd.ndeps = deps.size();
+            launch_build(p.k, bp, h, a, bd);
+            CKH(bd.err);
+            G.build[i] = bd.node;
+            Dispatch zd;
+            zd.graph = G.g;
+            zd(&k_zero, dim3(1), dim3(64), 0, a.cnt, CNT_WORDS);
+            CKH(zd.err);
+            std::vector<hipGraphNode_t> cdeps{G.build[i], zd.node};
+            if (i) cdeps.push_back(G.chain[i - 1]);
+            Dispatch cd;
+            cd.graph = G.g;
+            cd.deps = cdeps.data();
+            cd.ndeps = cdeps.size();
+            launch_chain(p.k, bgh, h, a, cd);
+            CKH(cd.err);
+            G.chain[i] = cd.node;
+        }
+        CKH(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
+        G.p = p;
+        G.ws = c.ws.base;
+        G.count = count;
+        G.layout = layout;
+        G.chunk = chunk;
+        G.nbuf = nbuf;
+    } else {
+        for (uint32_t i = 0; i < passes; ++i) {
+            const PassArgs a = pass_args(i);
+            Dispatch bd;
+            bd.exec = G.exec;
+            bd.node = G.build[i];
+            launch_build(p.k, bp, h, a, bd);
+            CKH(bd.err);
+            Dispatch cd;
+            cd.exec = G.exec;
+            cd.node = G.chain[i];
+            launch_chain(p.k, bgh, h, a, cd);
+            CKH(cd.err);
+        }
+    }
+    CKH(hipEventRecord(c.ev[0], c.build_stream));
+    CKH(hipGraphLaunch(G.exec, c.build_stream));
+    CKH(hipEventRecord(c.ev[1], c.build_stream));
+    CKH(hipEventSynchronize(c.ev[1]));
+    std::vector<uint32_t> cnt(size_t(passes) * 4);
+    CKH(hipMemcpy2D(cnt.data(), 16, G.aux, stride * 4, 16, passes, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < passes; ++i) {
+        const uint32_t n = std::min(chunk, count - i * chunk), nf = cnt[i * 4 + CNT_FAIL];
+        if (nf > n) return fail(CK_ERANGE, "device failure count out of range");
+        if (nf) {
+            const size_t at = out.fails->size();
+            out.fails->resize(at + nf);
+            CKH(hipMemcpy(out.fails->data() + at, G.aux + i * stride + CNT_WORDS, size_t(nf) * 4,
+                          hipMemcpyDeviceToHost));
+        }
+        if (st) st->probes += (uint64_t(cnt[i * 4 + CNT_PROBES + 1]) << 32) | cnt[i * 4 + CNT_PROBES];
+    }
+    if (st) {
+        float ms = 0;
+        CKH(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+        st->chain_ms += ms;
+        st->launches += 3 * passes;
+        st->graph = same ? 1 : 2;
+    }
+    return CK_OK;
+}
+
 // HBM tier: build + chain per pass of up to `chunk` trial slots.  Two buffers
 // and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
@@ -839,6 +1050,8 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         st->chunk = chunk;
     }
     const uint32_t passes = (count + chunk - 1) / chunk;
+    if (c.graph_on && !out.probes && !out.tables && !out.keys && !out.walks && !out.rnds && !out.map && !out.digests)
+        return run_hbm_graph(c, p, first, count, layout, nbuf, chunk, h, bp, args, out, st);
     auto enqueue = [&](uint32_t i) -> int {
         const int b = nbuf == 2 ? int(i & 1) : 0;
         PassArgs& a = args[b];
@@ -846,29 +1059,27 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         a.count = std::min(chunk, count - i * chunk);
         if (i >= 2) CKH(hipStreamWaitEvent(c.build_stream, c.ev_chain[b][1], 0));
         CKH(hipEventRecord(c.ev_build[b][0], c.build_stream));
-        switch (p.k) {
-            case 2: launch_build_k<2>(bp, h, a, c.build_stream); break;
-            case 3: launch_build_k<3>(bp, h, a, c.build_stream); break;
-            default: launch_build_k<4>(bp, h, a, c.build_stream); break;
-        }
+        Dispatch bd, cd;
+        bd.stream = c.build_stream;
+        cd.stream = c.stream;
+        launch_build(p.k, bp, h, a, bd);
+        CKH(bd.err);
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev_build[b][1], c.build_stream));
         CKH(hipStreamWaitEvent(c.stream, c.ev_build[b][1], 0));
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c.stream, a.cnt, CNT_WORDS);
+        cd(&k_zero, dim3(1), dim3(64), 0, a.cnt, CNT_WORDS);
         CKH(hipEventRecord(c.ev_chain[b][0], c.stream));
-        switch (p.k) {
-            case 2: launch_chain_k<2>(bgh, h, a, c.stream); break;
-            case 3: launch_chain_k<3>(bgh, h, a, c.stream); break;
-            default: launch_chain_k<4>(bgh, h, a, c.stream); break;
-        }
+        launch_chain(p.k, bgh, h, a, cd);
+        CKH(cd.err);
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev_chain[b][1], c.stream));
         if (a.digests) {
             switch (p.k) {
-                case 2: launch_digest_k<2>(h, a, c.stream); break;
-                case 3: launch_digest_k<3>(h, a, c.stream); break;
-                default: launch_digest_k<4>(h, a, c.stream); break;
+                case 2: launch_digest_k<2>(h, a, cd); break;
+                case 3: launch_digest_k<3>(h, a, cd); break;
+                default: launch_digest_k<4>(h, a, cd); break;
             }
+            CKH(cd.err);
             CKH(hipGetLastError());
         }
         return CK_OK;
@@ -1039,6 +1250,10 @@ int ck_ctx_create(int device, ck_ctx** out) {
 void ck_ctx_destroy(ck_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->c.device);
+    for (HbmGraph& g : ctx->c.graphs) {
+        g.drop();
+        if (g.aux) (void)hipFree(g.aux);
+    }
     if (ctx->c.ws.base) (void)hipFree(ctx->c.ws.base);
     for (auto& ev : ctx->c.ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1055,6 +1270,12 @@ void ck_ctx_destroy(ck_ctx* ctx) {
 int ck_ctx_set_tier(ck_ctx* ctx, uint32_t tier) {
     if (!ctx || tier > CK_TIER_LDS) return fail(CK_EINVAL, "bad tier");
     ctx->c.tier = tier;
+    return CK_OK;
+}
+
+int ck_ctx_set_graph(ck_ctx* ctx, uint32_t on) {
+    if (!ctx || on > 1) return fail(CK_EINVAL, "bad graph setting");
+    ctx->c.graph_on = on != 0;
     return CK_OK;
 }
 

@@ -34,6 +34,7 @@ void worker(int id, int device, const Options& o, const ck_params& p, const std:
             std::mutex& print_mutex) {
     Ctx ctx(device);
     check(ck_ctx_set_tier(ctx.h, tier_code(o.tier)), "ck_ctx_set_tier");
+    check(ck_ctx_set_graph(ctx.h, o.graph ? 1 : 0), "ck_ctx_set_graph");
     const uint64_t batches = total_batches(o);
     const uint32_t micro = static_cast<uint32_t>(std::min<uint64_t>(o.micro, 1ULL << o.bt));
     std::string tuned;
@@ -62,7 +63,7 @@ void worker(int id, int device, const Options& o, const ck_params& p, const std:
                   << " table=" << (o.tier == "auto" ? "auto" : o.tier) << " micro=" << micro << " k=" << p.k
                   << " stash=" << p.stash << " kick=" << p.kick << " half_bits=" << p.half_bits
                   << " CUs=" << prop.multiProcessorCount << " lds_per_block=" << prop.sharedMemPerBlock << tuned
-                  << "\n";
+                  << (o.graph ? " graph=on" : "") << "\n";
     }
     while (!s.stop.load(std::memory_order_relaxed)) {
         if (pause_requested(o)) {

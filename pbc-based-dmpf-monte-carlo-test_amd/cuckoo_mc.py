@@ -37,7 +37,7 @@ class Stats(C.Structure):
     _fields_ = [("trials", C.c_uint64), ("failed", C.c_uint64), ("probes", C.c_uint64),
                 ("tier", C.c_uint32), ("chunk", C.c_uint32), ("build_ms", C.c_double),
                 ("chain_ms", C.c_double), ("fused_ms", C.c_double), ("launches", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("graph", C.c_uint32)]
 
 
 class Job(C.Structure):
@@ -93,6 +93,7 @@ def _load():
         "ck_ctx_set_tier": (C.c_int, [vp, U32]),
         "ck_ctx_set_chunk": (C.c_int, [vp, U32]),
         "ck_ctx_set_layout": (C.c_int, [vp, U32]),
+        "ck_ctx_set_graph": (C.c_int, [vp, U32]),
         "ck_launch_digest": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U64),
                                        P(Stats)]),
         "ck_aes_self_test": (C.c_int, [vp]),
@@ -122,7 +123,7 @@ LIB = _load()
 
 # Every symbol declared in include/cuckoo_mc.h.
 EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "ck_ctx_destroy",
-           "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_ctx_set_layout", "ck_aes_self_test", "ck_launch",
+           "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_ctx_set_layout", "ck_ctx_set_graph", "ck_aes_self_test", "ck_launch",
            "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_ctx_tune", "ck_engine_create",
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
            "ck_engine_release", "ck_engine_stats_get"]
@@ -190,6 +191,9 @@ class Context:
 
     def set_layout(self, layout: int):
         _check(LIB.ck_ctx_set_layout(self.h, layout))
+
+    def set_graph(self, on: bool):
+        _check(LIB.ck_ctx_set_graph(self.h, 1 if on else 0))
 
     def self_test(self):
         _check(LIB.ck_aes_self_test(self.h))

@@ -43,11 +43,13 @@ def test_cli_golden_and_workers(tmp_path, fixtures, mode):
     for name, extra in [("baseline", ["--workers", "1", "--micro", "257"]),
                         ("multi", ["--workers", "2", "--micro", "37", "--tier", "hbm"]),
                         ("lds", ["--workers", "1", "--micro", "100", "--tier", "lds"]),
-        ("tuned", ["--workers", "1", "--micro", "256", "--tune"])]:
+                        ("tuned", ["--workers", "1", "--micro", "256", "--tune"]),
+                        ("graph", ["--workers", "2", "--micro", "100", "--graph"])]:
         d = tmp_path / name
         out = run(["--mode", mode, *SMOKE, *extra, "--stop-after", "8192", "--out", str(d)])
         assert "RESULT status=complete" in out.stdout
         assert ("tuned=" in out.stderr) == (name == "tuned")
+        assert ("graph=on" in out.stderr) == (name == "graph")
         s = summary(d)
         assert s["status"] == "complete" and s["completed"] == 8192 and s["completed_batches"] == 32
         assert s["failed"] == len(want)
