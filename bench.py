@@ -37,7 +37,11 @@ SPLITMIX_OPS = 22      # one SplitMix64 draw (int32-equivalent)
 CACHED_ROUND_OPS = 4   # one cached Feistel round (lookup index, xor, mask, swap)
 EVAL_FIXED_OPS = 6     # element, y = k*e+w, cycle-walk test, division
 PROBE_OPS = 20         # choice draw (div/mod by k), table read, compare, write, advance
-SLOT_BYTES = 8         # one probe reads a 4-byte slot and writes at most one
+# Algorithmic bytes of k_chain in the descriptor layout (8-byte slots): a probe
+# reads its slot and writes it back (placement or eviction), and each new item's
+# 8-byte descriptor is read once from the insertion-order stream.
+PROBE_BYTES = 16
+ITEM_BYTES = 8
 # HBM bytes per probe of k_chain in the descriptor layout with load + store
 # probes (the default at this shape), from FETCH_SIZE and WRITE_SIZE per
 # dispatch divided by the dispatch's probes; the chain's reads are 64-byte
@@ -352,7 +356,8 @@ def hbm_leg(ck, trials: int, layout: int = 0):
         # Chains are latency-bound, so a pass takes about as long whatever its trial
         # count: a long run of full passes goes at trials_per_pass / pass time.
         "full_pass_trials_per_s": st.chunk / pass_s if pass_s > 0 else None,
-        "chain_algorithmic_gbs": SLOT_BYTES * st.probes / chain_s / 1e9 if chain_s > 0 else None,
+        "chain_algorithmic_gbs": ((PROBE_BYTES * st.probes + ITEM_BYTES * p.pnum * trials) / chain_s / 1e9
+                                  if chain_s > 0 else None),
         "device_ms": {"build": st.build_ms, "chain": st.chain_ms},
     }
 
@@ -389,8 +394,8 @@ def main():
                     help="before warmup, copy a buffer of this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(trial batches would shard across GPUs by batch id; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(trial batches would shard across GPUs by batch id)", file=sys.stderr)
         return 2
     t = args.trials
     if t <= 0 or t & (t - 1) or t > (1 << BT):
@@ -454,7 +459,7 @@ def main():
     value = trials / elapsed
     chain_launches = max(1, agg["launches"] // 2)  # build + chain per pass
     chain_avg_s = agg["chain_ms"] / 1e3 / chain_launches
-    bytes_per_launch = SLOT_BYTES * agg["probes"] / chain_launches
+    bytes_per_launch = (PROBE_BYTES * agg["probes"] + ITEM_BYTES * p.pnum * trials) / chain_launches
     achieved = bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else 0.0
     measured_layout = args.layout in ("auto", "desc-plain")
     traffic = ((TRAFFIC_READ_PER_PROBE + TRAFFIC_WRITE_PER_PROBE) * agg["probes"] / chain_launches
@@ -488,8 +493,9 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "avg_launch_ms": chain_avg_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
             "traffic_rate_gbs": traffic / chain_avg_s / 1e9 if traffic and chain_avg_s > 0 else None,
-            "note": "achieved counts 8 algorithmic bytes per probe; each probe costs one random 64-byte DRAM "
-                    "read and ~0.86 32-byte write-backs, and the chain is bound by that random-access rate",
+            "note": "achieved counts 16 algorithmic bytes per probe (8-byte slot read and write) and 8 per "
+                    "inserted item (its descriptor); each probe costs one random 64-byte DRAM read and ~0.86 "
+                    "32-byte write-backs, and the chain is bound by that random-access rate",
         },
         "int_ops_model": {
             "ops_per_trial": ops, "achieved_tops": ops * value / 1e12, "peak_tops": VALU_PEAK_TOPS,
@@ -512,9 +518,6 @@ def main():
         best = max(ents, key=lambda e: e["trials_per_s"])
         print(f"autotune: best tier {best['tier']} layout {names[best['layout']]} chunk {best['chunk']} "
               f"{best['trials_per_s']:.0f} trials/s", file=sys.stderr, flush=True)
-    if not args.no_graph:
-        rec["graph"] = graph_leg(ck, p, args.steps, t)
-        ok = ok and all(g["results_equal"] for g in rec["graph"])
     if not args.no_cpu:
         rec["config1"], good = config1_leg(ck)
         ok = ok and good
@@ -528,6 +531,11 @@ def main():
         lo, hi = (float(x) for x in args.serve_budget.split(","))
         rec["serving"], good = serve_leg(ck, [float(r) for r in args.serve_rates.split(",")], args.serve_jobs, lo, hi)
         ok = ok and good
+    if not args.no_graph:
+        # Last: a graph-mode context leaves its workspace allocated until the
+        # process ends, which would shrink the passes of any later leg.
+        rec["graph"] = graph_leg(ck, p, args.steps, t)
+        ok = ok and all(g["results_equal"] for g in rec["graph"])
     print(json.dumps(rec), flush=True)
     return 0 if ok else 1
 

@@ -75,6 +75,14 @@ __device__ __forceinline__ void build_cache(const LdsTab& T, const TrialSeed& ts
     wave_lds_sync();
 }
 
+// The build's outputs -- bucket-map rows and the cleared table -- are written
+// once and read by the chain kernel only after the whole build, so they are
+// stored non-temporally: streamed out without displacing the lines that the
+// chains of the other pass, running at the same time, are reading and writing
+// back.  Measured at n = 2^14 with the build overlapping the chains: chain
+// 99.3 -> 96.1 ms per 65,536-trial pass; a second, ordinary clear of the table
+// costs 2.8 ms of chain time.
+
 // The bucket map row of ordinal o: k entries, or one packed descriptor.
 template <int K, class M>
 __device__ __forceinline__ void store_candidates(M* m, const Shape& S, uint32_t o, const uint32_t (&c)[K]) {
@@ -82,11 +90,20 @@ __device__ __forceinline__ void store_candidates(M* m, const Shape& S, uint32_t 
         uint64_t d = 0;
 #pragma unroll
         for (int w = 0; w < K; ++w) d |= uint64_t(c[w]) << (w * S.dbits);
-        m[o] = d;
+        __builtin_nontemporal_store(d, &m[o]);
     } else {
 #pragma unroll
-        for (int w = 0; w < K; ++w) m[o * K + w] = static_cast<M>(c[w]);
+        for (int w = 0; w < K; ++w) __builtin_nontemporal_store(static_cast<M>(c[w]), &m[o * K + w]);
     }
+}
+
+// Fills a table of `bytes` (a multiple of 16) with all-ones (empty), the
+// wave's 64 lanes storing 16 bytes each per step.
+__device__ __forceinline__ void clear_table(char* table, size_t bytes, uint32_t lane) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4* t = reinterpret_cast<u32x4*>(table);
+    const u32x4 e = {EMPTY, EMPTY, EMPTY, EMPTY};
+    for (size_t i = lane; i < bytes / 16; i += 64) __builtin_nontemporal_store(e, &t[i]);
 }
 
 }  // namespace ck

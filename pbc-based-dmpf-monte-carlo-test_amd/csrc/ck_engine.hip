@@ -102,9 +102,7 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build_mix(const SegDev* 
             store_candidates<K>(m, S, o, c);
         }
     }
-    uint4* t4 = reinterpret_cast<uint4*>(arena + G.table_off + size_t(i) * G.table_bytes);
-    const uint4 e4 = make_uint4(EMPTY, EMPTY, EMPTY, EMPTY);
-    for (uint32_t j = lane; j < G.table_bytes / 16; j += 64) t4[j] = e4;
+    clear_table(arena + G.table_off + size_t(i) * G.table_bytes, G.table_bytes, lane);
     if (lane == 0) {
         walks[slot] = ts.walk;
         rnds[slot] = ts.rnd;

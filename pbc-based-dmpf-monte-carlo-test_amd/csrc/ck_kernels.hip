@@ -90,9 +90,7 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys
         }
     }
 
-    uint4* t4 = reinterpret_cast<uint4*>(tables + slot * table_bytes);
-    const uint4 e4 = make_uint4(EMPTY, EMPTY, EMPTY, EMPTY);
-    for (uint32_t i = lane; i < table_bytes / 16; i += 64) t4[i] = e4;
+    clear_table(tables + slot * table_bytes, table_bytes, lane);
     if (lane == 0) {
         walks[slot] = ts.walk;
         rnds[slot] = ts.rnd;

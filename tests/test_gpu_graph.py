@@ -80,15 +80,19 @@ def test_graph_cache_rebuilds_on_new_count_and_shape():
 
 
 def test_graph_full_batch_matches_at_headline_shape():
-    """A whole 2^17-trial range at the headline shape in the library's default
-    passes (two 65,536-trial passes, double-buffered), replayed twice."""
+    """A 2^16-trial range at the headline shape in four double-buffered passes
+    of 16,384 trials, replayed twice.  (The pass size is below the library's
+    default to bound the workspace that a graph-mode context leaves allocated
+    in this process.)"""
     p = ck.params("bgh25", 16384, 16, kick=500)
     with ck.Context(0) as g, ck.Context(0) as e:
         g.set_graph(True)
+        g.set_chunk(1 << 14)
         for batch in (0, 1):
             rk = ck.batch_key(1, batch)
-            f, st = g.launch(p, rk, 0, 1 << 17)
-            ef, est = e.launch(p, rk, 0, 1 << 17)
+            f, st = g.launch(p, rk, 0, 1 << 16)
+            ef, est = e.launch(p, rk, 0, 1 << 16)
             assert st.graph == (2 if batch == 0 else 1)
             assert list(map(int, f)) == list(map(int, ef))
             assert st.probes == est.probes
+
