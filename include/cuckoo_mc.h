@@ -122,7 +122,7 @@ typedef struct ck_tune_entry {
     double trials_per_s;          /* wall-clock rate on the sample */
 } ck_tune_entry;
 
-/* Autotuner (the north star's per-shape tuning of the placement that the
+/* Autotuner (per-shape tuning of the placement that the
  * reference exposes as --table-layout/--shared-variant/--block-threads,
  * cuckoo_bench.cu:819-901): runs trials [0, sample) of the batch under every
  * applicable tier / table layout / trials-per-pass candidate, checks that all
