@@ -159,18 +159,28 @@ __device__ __forceinline__ bool chain_drrt(const Shape& S, const M* __restrict__
 
 // ---------------------------------------------------------------------------
 // Descriptor layout (HBM tier, when k*bb + 2 <= 63 with bb = bit width of the
-// largest bucket index): a table entry holds the occupant's k bucket indices
-// and its current choice, desc | choice << (k*bb), and all-ones is empty.  An
-// eviction therefore reads the occupant's candidates from the slot it just
-// loaded instead of from the bucket map; new items take their descriptors from
-// a per-trial stream in insertion order, prefetched one item ahead.
+// largest bucket index): a table entry holds the occupant's k bucket indices,
+// its current choice and the pass's tag,
+//   desc | choice << (k*bb) | tag << (k*bb + 2),
+// and an entry whose tag is not the pass's is empty.  A cleared table is
+// all-ones, whose tag field (all-ones) no pass uses; a table reused without a
+// clear holds only entries of earlier tags.  An eviction reads the occupant's
+// candidates from the slot it just loaded instead of from the bucket map; new
+// items take their descriptors from a per-trial stream in insertion order,
+// prefetched one item ahead.
 struct DescCodec {
     uint32_t bb;        // bits per bucket index
     uint32_t cs;        // choice shift = k*bb
     uint64_t bmask;     // (1 << bb) - 1
     uint64_t dmask;     // (1 << cs) - 1
+    uint32_t ts = cs + 2;  // tag shift
+    uint64_t tag = 0;      // this pass's tag (0 after a clear)
     __device__ __forceinline__ uint32_t bucket(uint64_t d, uint32_t w) const {
         return static_cast<uint32_t>((d >> (w * bb)) & bmask);
+    }
+    __device__ __forceinline__ bool empty(uint64_t e) const { return (e >> ts) != tag; }
+    __device__ __forceinline__ uint64_t entry(uint64_t d, uint32_t w) const {
+        return d | (uint64_t(w) << cs) | (tag << ts);
     }
 };
 constexpr uint64_t EMPTY64 = ~0ull;
@@ -189,7 +199,7 @@ __device__ __forceinline__ bool chain_bgh_desc(const Shape& S, const DescCodec& 
         const uint32_t b = D.bucket(cur, which);
         const uint64_t old = t[b];
         ++pr;
-        bool advance = old == EMPTY64;
+        bool advance = D.empty(old);
         if (!advance && ++coll > S.kick) {
             if (!stash_left) {
                 failed = true;
@@ -198,7 +208,7 @@ __device__ __forceinline__ bool chain_bgh_desc(const Shape& S, const DescCodec& 
             --stash_left;  // the item in hand goes to the stash; nothing is written
             advance = true;
         } else {
-            t[b] = cur | (uint64_t(which) << D.cs);
+            t[b] = D.entry(cur, which);
         }
         if (advance) {
             ++item;
@@ -228,8 +238,8 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
             if (placed) break;
             ++pr;
             const uint32_t b = D.bucket(d, w);
-            if (t[b] == EMPTY64) {
-                t[b] = d | (uint64_t(w) << D.cs);
+            if (D.empty(t[b])) {
+                t[b] = D.entry(d, w);
                 placed = true;
             }
         }
@@ -239,8 +249,8 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
         for (;;) {
             const uint64_t disp = t[target];
             ++pr;
-            if (disp == EMPTY64) {
-                t[target] = cur | (uint64_t(choice) << D.cs);
+            if (D.empty(disp)) {
+                t[target] = D.entry(cur, choice);
                 break;
             }
             if (kicks >= S.kick) {
@@ -251,7 +261,7 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
                 failed = true;
                 break;
             }
-            t[target] = cur | (uint64_t(choice) << D.cs);
+            t[target] = D.entry(cur, choice);
             cur = disp & D.dmask;
             choice = static_cast<uint32_t>(disp >> D.cs) & 3u;
             const uint32_t prev = target;

@@ -50,14 +50,16 @@ __device__ __forceinline__ LdsTab stage_tab(uint32_t* lds, uint32_t shift) {
 // build (HBM tier): one wave per trial slot.  CACHED selects the round-cache
 // form (half <= 12); otherwise every bucket evaluation runs AES directly.
 // M is the map entry width (u16 when buckets <= 65536), or u64 for the
-// descriptor layout (one packed candidate list per ordinal); the slot's table
-// (table_bytes, a multiple of 16) is filled with all-ones (empty).
+// descriptor layout (one packed candidate list per ordinal).  With `clear`, the
+// slot's table (table_bytes, a multiple of 16) is filled with all-ones (empty);
+// without, the chain's pass tag makes the previous passes' entries read as
+// empty.
 template <int K, bool CACHED, class M>
 __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
                                                uint32_t tab_shift, M* __restrict__ map, size_t map_stride,
                                                char* __restrict__ tables, size_t table_bytes,
                                                uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
-                                               uint32_t* __restrict__ key_out) {
+                                               uint32_t* __restrict__ key_out, uint32_t clear) {
     extern __shared__ uint32_t lds[];
     const LdsTab T = stage_tab(lds, tab_shift);
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys
         }
     }
 
-    clear_table(tables + slot * table_bytes, table_bytes, lane);
+    if (clear) clear_table(tables + slot * table_bytes, table_bytes, lane);
     if (lane == 0) {
         walks[slot] = ts.walk;
         rnds[slot] = ts.rnd;
@@ -98,15 +100,16 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys
 }
 
 // chain (HBM tier): one lane per trial slot; map and table widths M and E
-// (M = u64 selects the descriptor layout).  XCHG selects one-round-trip
-// probes (atomic exchange / compare-and-swap; needs 32- or 64-bit entries).
+// (M = u64 selects the descriptor layout, whose entries carry `tag`).  XCHG
+// selects one-round-trip probes (atomic exchange / compare-and-swap; needs 32-
+// or 64-bit entries; always on cleared tables, tag 0).
 template <int K, bool BGH, class M, class E, bool XCHG>
 __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t count,
                                                const M* __restrict__ map, size_t map_stride,
                                                char* __restrict__ tables, size_t table_bytes,
                                                const uint64_t* __restrict__ walks, const uint64_t* __restrict__ rnds,
                                                uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
-                                               uint32_t* __restrict__ probes_out) {
+                                               uint32_t* __restrict__ probes_out, uint64_t tag) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     bool failed = false;
     uint32_t pr = 0;
@@ -114,7 +117,8 @@ __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t
         const M* m = map + slot * map_stride;
         E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
         if constexpr (sizeof(M) == 8) {
-            const DescCodec D{S.dbits, K * S.dbits, (1ull << S.dbits) - 1, (1ull << (K * S.dbits)) - 1};
+            const DescCodec D{S.dbits, K * S.dbits, (1ull << S.dbits) - 1, (1ull << (K * S.dbits)) - 1,
+                              K * S.dbits + 2, XCHG ? 0ull : tag};
             if constexpr (XCHG) {
                 const DescView<K> v{D, m};
                 if constexpr (BGH)
@@ -417,6 +421,15 @@ struct Ctx {
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
     uint32_t cus = 1;
+    // Descriptor tables reused without a clear: the plan of the tables in the
+    // workspace, and per pass buffer whether the tables of its first `slots`
+    // slots hold only entries of that plan written with tags up to `tag`.
+    uint64_t table_plan = 0;
+    struct {
+        bool valid;
+        uint32_t slots;
+        uint64_t tag;
+    } table_tags[2] = {};
     bool graph_on = false;  // HBM-tier launches run as replayed graphs
     static constexpr int kGraphs = 4;  // graphs kept per context, least recently used replaced first
     HbmGraph graphs[kGraphs];
@@ -629,8 +642,14 @@ static uint32_t lds_chunk(const Ctx& c, uint32_t count, bool tables, const ck_pa
     return std::max<uint32_t>(1, std::min(target, count));
 }
 
+static void forget_tables(Ctx& c) {
+    c.table_plan = 0;
+    c.table_tags[0] = c.table_tags[1] = {};
+}
+
 static int ensure_ws(Ctx& c, size_t bytes) {
     if (c.ws.bytes >= bytes) return CK_OK;
+    forget_tables(c);
     if (c.ws.base) CKH(hipFree(c.ws.base));
     c.ws.base = nullptr;
     c.ws.bytes = 0;
@@ -654,6 +673,8 @@ struct PassArgs {
     uint32_t* probes;  // null unless requested
     uint32_t* keys;    // null unless requested
     unsigned long long* digests;  // null unless requested
+    uint32_t clear;    // the build clears the tables
+    uint64_t tag;      // descriptor-entry tag of the pass
 };
 
 template <int K, class M>
@@ -662,10 +683,10 @@ static void launch_build_t(const BuildPlan& bp, const PassArgs& a, Dispatch& st)
     M* map = reinterpret_cast<M*>(a.map);
     if (bp.cached)
         st(&k_build<K, true, M>, dim3(blocks), dim3(64 * bp.waves), bp.lds, a.S, a.bk, a.first, a.count, kRepl, map,
-           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
+           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys, a.clear);
     else
         st(&k_build<K, false, M>, dim3(blocks), dim3(64 * bp.waves), bp.lds, a.S, a.bk, a.first, a.count, kRepl, map,
-           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys);
+           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys, a.clear);
 }
 template <int K>
 static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs& a, Dispatch& st) {
@@ -682,7 +703,7 @@ static void launch_chain_t(const PassArgs& a, Dispatch& st) {
     const uint32_t blocks = (a.count + 255) / 256;
     st(&k_chain<K, BGH, M, E, XCHG>, dim3(blocks), dim3(256), 0, a.S, a.first, a.count,
        reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.cnt, a.fail,
-       a.probes);
+       a.probes, a.tag);
 }
 template <int K, class M, class E>
 static void launch_digest_t(const PassArgs& a, Dispatch& st) {
@@ -864,6 +885,39 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
     return CK_OK;
 }
 
+// Table reuse without a clear (descriptor layout with load + store probes).
+// Entries carry the tag of the pass that wrote them in the bits above the
+// choice, and entries of any other tag read as empty.  A buffer's tables
+// therefore need clearing only on first use, when the plan of the tables in
+// the workspace changes (position, size, tag position), when the tag field is
+// used up (all-ones, the cleared value, is never a tag), and when a pass uses
+// more slots than the buffer's last clear covered (a clear covers the slots of
+// its own pass only).  A clear writes 192 KB per trial at n = 2^14, traffic
+// that slows the other pass's chains.
+static uint64_t table_plan(const Ctx& c, const ck_params& p, const HbmPlan& h, uint32_t nbuf, bool taggable) {
+    uint64_t x = reinterpret_cast<uintptr_t>(c.ws.base);
+    for (uint64_t v : {uint64_t(h.buffer), uint64_t(h.off_tables), uint64_t(h.table_bytes), uint64_t(nbuf),
+                       uint64_t(p.k) * make_shape(p).dbits, uint64_t(taggable)}) {
+        x = (x ^ v) * 0x9E3779B97F4A7C15ull;
+        x ^= x >> 29;
+    }
+    return x | 1;  // 0 means no plan
+}
+static void pass_tags(Ctx& c, uint32_t b, bool taggable, uint32_t tag_bits, PassArgs& a) {
+    auto& T = c.table_tags[b];
+    const uint64_t max_tag = (uint64_t(1) << std::min<uint32_t>(tag_bits, 63)) - 2;
+    if (taggable && T.valid && a.count <= T.slots && T.tag < max_tag) {
+        a.clear = 0;
+        a.tag = ++T.tag;
+        return;
+    }
+    a.clear = 1;
+    a.tag = 0;
+    T.valid = taggable;
+    T.slots = a.count;
+    T.tag = 0;
+}
+
 // Graph mode of the HBM tier.  The first call for a (shape, trial count,
 // layout, pass size, workspace) builds a graph of every pass -- build, counter
 // clear and chain nodes, with the dependencies the two streams give in the
@@ -874,8 +928,8 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
 // whole graph has run.  Device time is reported as chain_ms for the whole
 // replay (build_ms stays 0: the kernels of a graph are not timed one by one).
 static int run_hbm_graph(Ctx& c, const ck_params& p, uint32_t first, uint32_t count, uint32_t layout, uint32_t nbuf,
-                         uint32_t chunk, const HbmPlan& h, const BuildPlan& bp, PassArgs (&args)[2], HostOut& out,
-                         ck_stats* st) {
+                         uint32_t chunk, const HbmPlan& h, const BuildPlan& bp, PassArgs (&args)[2], bool taggable,
+                         uint32_t tag_bits, HostOut& out, ck_stats* st) {
     const uint32_t passes = (count + chunk - 1) / chunk;
     const size_t stride = (CNT_WORDS + size_t(chunk) + 63) & ~size_t(63);  // u32 words per pass, 256-byte aligned
     const bool bgh = p.mode == CK_MODE_BGH25;
@@ -901,12 +955,15 @@ static int run_hbm_graph(Ctx& c, const ck_params& p, uint32_t first, uint32_t co
         }
     }
     // Pass i's arguments: its buffer's regions, its trial range, its counters.
+    // Called once per pass, in pass order, for every build or replay.
     auto pass_args = [&](uint32_t i) {
-        PassArgs a = args[nbuf == 2 ? (i & 1) : 0];
+        const uint32_t b = nbuf == 2 ? (i & 1) : 0;
+        PassArgs a = args[b];
         a.first = first + i * chunk;
         a.count = std::min(chunk, count - i * chunk);
         a.cnt = G.aux + i * stride;
         a.fail = a.cnt + CNT_WORDS;
+        pass_tags(c, b, taggable, tag_bits, a);
         return a;
     };
     if (!same) {
@@ -1048,13 +1105,21 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         st->chunk = chunk;
     }
     const uint32_t passes = (count + chunk - 1) / chunk;
+    const bool taggable = layout == CK_LAYOUT_DESC_PLAIN && !out.digests;
+    const uint32_t tag_bits = taggable ? 62 - p.k * args[0].S.dbits : 0;
+    const uint64_t plan = table_plan(c, p, h, nbuf, taggable);
+    if (plan != c.table_plan) {
+        forget_tables(c);
+        c.table_plan = plan;
+    }
     if (c.graph_on && !out.probes && !out.tables && !out.keys && !out.walks && !out.rnds && !out.map && !out.digests)
-        return run_hbm_graph(c, p, first, count, layout, nbuf, chunk, h, bp, args, out, st);
+        return run_hbm_graph(c, p, first, count, layout, nbuf, chunk, h, bp, args, taggable, tag_bits, out, st);
     auto enqueue = [&](uint32_t i) -> int {
         const int b = nbuf == 2 ? int(i & 1) : 0;
         PassArgs& a = args[b];
         a.first = first + i * chunk;
         a.count = std::min(chunk, count - i * chunk);
+        pass_tags(c, b, taggable, tag_bits, a);
         if (i >= 2) CKH(hipStreamWaitEvent(c.build_stream, c.ev_chain[b][1], 0));
         CKH(hipEventRecord(c.ev_build[b][0], c.build_stream));
         Dispatch bd, cd;
@@ -1141,6 +1206,7 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
                    ck_stats* st) {
     const LdsPlan plan = plan_lds(p, c.lds_block, c.lds_cu);
     if (!plan.ok) return fail(CK_EINVAL, "shape does not fit the LDS tier (half_bits > 12 or table too large)");
+    forget_tables(c);  // the workspace becomes this tier's scratch
     const Shape S = make_shape(p);
     RoundKeys bk;
     std::memcpy(bk.w, rk, sizeof bk.w);
