@@ -46,7 +46,7 @@ ITEM_BYTES = 8
 # probes (the default at this shape), from FETCH_SIZE and WRITE_SIZE per
 # dispatch divided by the dispatch's probes; the chain's reads are 64-byte
 # requests, which FETCH_SIZE counts exactly (profiles/k_chain_hbm_n16384_desc_traffic.md).
-TRAFFIC_READ_PER_PROBE = 65.07
+TRAFFIC_READ_PER_PROBE = 64.45
 TRAFFIC_WRITE_PER_PROBE = 27.48
 
 
