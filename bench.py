@@ -335,13 +335,16 @@ def graph_leg(ck, p, steps: int, trials: int):
     return out
 
 
-def hbm_leg(ck, trials: int, layout: int = 0):
+def hbm_leg(ck, trials: int, layout: int = 0, warm: int = 0):
     """Rare-event HBM shape: n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
-    ell_i 20, seed 4 -- the first `trials` trials of batch 0."""
+    ell_i 20, seed 4 -- the first `trials` trials of batch 0, after an untimed
+    launch of `warm` trials of batch 1 (default: as many, so that the
+    workspace, 95 % of the free memory, is allocated outside the timed region;
+    a 64-trial warm-up allocates a small one)."""
     p = ck.params("bgh25", 1 << 20, 20, k=3, buckets=1363149, kick=1000, stash=2)
     ctx = ck.Context(0)
     ctx.set_layout(layout)
-    ctx.launch(p, ck.batch_key(4, 0), 0, 64)  # workspace allocation outside the timed region
+    ctx.launch(p, ck.batch_key(4, 1), 0, warm or trials)
     t0 = time.perf_counter()
     f, st = ctx.launch(p, ck.batch_key(4, 0), 0, trials)
     dt = time.perf_counter() - t0
@@ -409,7 +412,8 @@ def main():
     layout = {"auto": ck.LAYOUT_AUTO, "map": ck.LAYOUT_MAP, "desc": ck.LAYOUT_DESC, "map-xchg": ck.LAYOUT_MAP_XCHG,
               "desc-plain": ck.LAYOUT_DESC_PLAIN, "residual": ck.LAYOUT_RESIDUAL}[args.layout]
     if args.hbm_only:
-        print(json.dumps({"hbm_tier": hbm_leg(ck, args.hbm_trials, layout)}), flush=True)
+        # Counter profiles: a 64-trial warm-up keeps the dispatches to profile few.
+        print(json.dumps({"hbm_tier": hbm_leg(ck, args.hbm_trials, layout, warm=64)}), flush=True)
         return 0
     p = ck.params(SHAPE["mode"], SHAPE["pnum"], SHAPE["ell"], kick=SHAPE["kick"])
     ctx = ck.Context(0)
