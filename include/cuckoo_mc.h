@@ -53,8 +53,8 @@ enum {
 /* HBM-tier table layout and probe form. */
 enum {
     CK_LAYOUT_AUTO = 0,       /* DESC_PLAIN when it fits and tables are not requested; else RESIDUAL
-                                 when memory caps the pass (single-buffered); else MAP_XCHG below 2^14
-                                 trials per pass and MAP above */
+                                 when it fits ((k-1)*bits+2 <= 63); else MAP_XCHG below 2^14 trials
+                                 per pass and MAP above */
     CK_LAYOUT_MAP = 1,        /* entry = ordinal | choice (u16 when it fits); load + store probes */
     CK_LAYOUT_DESC = 2,       /* entry = occupant's packed candidates | choice (k*bits+2 <= 63); exchange probes */
     CK_LAYOUT_MAP_XCHG = 3,   /* u32 entry = ordinal | choice; exchange probes */

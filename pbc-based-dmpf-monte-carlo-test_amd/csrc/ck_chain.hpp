@@ -75,11 +75,21 @@ __device__ __forceinline__ bool chain_bgh(const Shape& S, const M* __restrict__ 
     return failed;
 }
 
+// An item's K <= 4 bucket indices, as one vector value.  A plain array would be
+// a stack object: an index known only at run time (pick(), or a loop the
+// optimiser has not unrolled yet) keeps it in memory, and the backend then
+// places it in LDS (K*4 bytes per lane).  A chain kernel holding LDS cannot
+// share a CU with a 160 KB build block, which slowed overlapped passes
+// 2.5-fold.  A vector stays in registers.
+typedef uint32_t Cands __attribute__((ext_vector_type(4)));
+
+// c[i] for a run-time i < K, as selects on the vector's elements.
 template <int K>
-__device__ __forceinline__ uint32_t pick(const uint32_t (&c)[K], uint32_t i) {
-    uint32_t v = c[0];
-#pragma unroll
-    for (int w = 1; w < K; ++w) v = (i == uint32_t(w)) ? c[w] : v;
+__device__ __forceinline__ uint32_t pick(const Cands& c, uint32_t i) {
+    uint32_t v = c.x;
+    if constexpr (K > 1) v = (i == 1u) ? c.y : v;
+    if constexpr (K > 2) v = (i == 2u) ? c.z : v;
+    if constexpr (K > 3) v = (i == 3u) ? c.w : v;
     return v;
 }
 
@@ -94,7 +104,7 @@ __device__ __forceinline__ bool chain_drrt(const Shape& S, const M* __restrict__
     uint32_t pr = 0, stash_left = S.stash;
     bool failed = false;
     for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
-        uint32_t c[K];
+        Cands c = {};
 #pragma unroll
         for (int w = 0; w < K; ++w) c[w] = m[item * K + w];
         bool placed = false;
@@ -128,7 +138,7 @@ __device__ __forceinline__ bool chain_drrt(const Shape& S, const M* __restrict__
             cur = entry_ordinal<E>(disp);
             choice = entry_choice<E>(disp);
             const uint32_t prev = target;
-            uint32_t cc[K];
+            Cands cc = {};
 #pragma unroll
             for (int w = 0; w < K; ++w) cc[w] = m[cur * K + w];
             bool found = false;
@@ -308,7 +318,7 @@ struct ResCodec {
     uint32_t bb;     // bits per bucket index
     uint32_t cs;     // choice shift = (K-1)*bb
     uint64_t bmask;  // (1 << bb) - 1
-    __device__ __forceinline__ E enc(const uint32_t (&c)[K], uint32_t w) const {
+    __device__ __forceinline__ E enc(const Cands& c, uint32_t w) const {
         uint64_t e = uint64_t(w) << cs;
 #pragma unroll
         for (int i = 0; i < K; ++i) {
@@ -318,7 +328,7 @@ struct ResCodec {
         return static_cast<E>(e);
     }
     // Candidates of the occupant of bucket b; returns its choice.
-    __device__ __forceinline__ uint32_t dec(E e, uint32_t b, uint32_t (&c)[K]) const {
+    __device__ __forceinline__ uint32_t dec(E e, uint32_t b, Cands& c) const {
         const uint64_t v = e;
         const uint32_t w = static_cast<uint32_t>(v >> cs) & 3u;
 #pragma unroll
@@ -337,7 +347,7 @@ struct CandStream {
     const M* m;
     uint32_t bb;
     uint64_t bmask;
-    __device__ __forceinline__ void load(uint32_t item, uint32_t (&c)[K]) const {
+    __device__ __forceinline__ void load(uint32_t item, Cands& c) const {
         if constexpr (sizeof(M) == 8) {
             const uint64_t d = m[item];
 #pragma unroll
@@ -355,7 +365,7 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
                                               E* __restrict__ t, uint64_t walk, uint64_t rnd, uint32_t& probes) {
     constexpr E kEmpty = static_cast<E>(~E(0));
     uint32_t item = 0, coll = 0, stash_left = S.stash, pr = 0;
-    uint32_t c[K], nx[K];
+    Cands c = {}, nx = {};
     st.load(0, c);
     if (S.pnum > 1) st.load(1, nx);
     bool failed = false;
@@ -378,8 +388,7 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
         }
         if (advance) {
             ++item;
-#pragma unroll
-            for (int w = 0; w < K; ++w) c[w] = nx[w];
+            c = nx;
             if (item + 1 < S.pnum) st.load(item + 1, nx);
             coll = 0;
         } else {
@@ -397,12 +406,10 @@ __device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K,
     constexpr E kEmpty = static_cast<E>(~E(0));
     uint32_t pr = 0, stash_left = S.stash;
     bool failed = false;
-    uint32_t nx[K];
+    Cands nx = {};
     st.load(0, nx);
     for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
-        uint32_t c[K];
-#pragma unroll
-        for (int w = 0; w < K; ++w) c[w] = nx[w];
+        Cands c = nx;
         if (item + 1 < S.pnum) st.load(item + 1, nx);
         bool placed = false;
 #pragma unroll

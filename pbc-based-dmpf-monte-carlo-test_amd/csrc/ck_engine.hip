@@ -111,10 +111,11 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build_mix(const SegDev* 
 
 // Table layouts of a job (its group keeps one per pass):
 //   LAY_MAP  u32 bucket-map rows, u32 entries ordinal | choice << 30;
-//   LAY_DESC u64 packed candidate stream, u64 entries candidates | choice.
-// (Residual entries, the single-shape path's third form, measured slower here:
-// their 36-38 VGPR chain waves crowd the builds that share the CUs.)
-constexpr uint32_t LAY_MAP = 0, LAY_DESC = 1;
+//   LAY_DESC u64 packed candidate stream, u64 entries candidates | choice;
+//   LAY_RES  u64 packed candidate stream, u64 entries holding the occupant's
+//            other k-1 candidates | choice (its current bucket is the slot):
+//            one random read per probe where the descriptor is too wide.
+constexpr uint32_t LAY_MAP = 0, LAY_DESC = 1, LAY_RES = 2;
 
 // chain: one wave per segment, one lane per trial.
 template <int K, bool BGH, uint32_t LAY>
@@ -139,6 +140,15 @@ __global__ __launch_bounds__(256) void k_chain_mix(const SegDev* __restrict__ se
                 failed = chain_bgh_desc<K>(S, D, m, t, walks[slot], rnds[slot], pr);
             else
                 failed = chain_drrt_desc<K>(S, D, m, t, pr);
+        } else if constexpr (LAY == LAY_RES) {
+            const uint64_t* m = reinterpret_cast<const uint64_t*>(arena + G.map_off + size_t(lane) * G.map_bytes);
+            uint64_t* t = reinterpret_cast<uint64_t*>(arena + G.table_off + size_t(lane) * G.table_bytes);
+            const ResCodec<K, uint64_t> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+            const CandStream<K, uint64_t> st{m, S.dbits, (1ull << S.dbits) - 1};
+            if constexpr (BGH)
+                failed = chain_bgh_res<K>(S, R, st, t, walks[slot], rnds[slot], pr);
+            else
+                failed = chain_drrt_res<K>(S, R, st, t, pr);
         } else {
             const uint32_t* m = reinterpret_cast<const uint32_t*>(arena + G.map_off + size_t(lane) * G.map_bytes);
             uint32_t* t = reinterpret_cast<uint32_t*>(arena + G.table_off + size_t(lane) * G.table_bytes);
@@ -401,7 +411,7 @@ static bool plan_pass(Engine& E, int bi, int& rc) {
 }
 
 template <int K>
-static void launch_build(Engine& E, const PassBuf& B, bool cached, bool desc, uint32_t max_half) {
+static void launch_build(Engine& E, const PassBuf& B, bool cached, bool stream64, uint32_t max_half) {
     const uint32_t slots = B.nsegs * SEG;
     const size_t tab = tab_bytes(kRepl);
     if (cached) {
@@ -410,7 +420,7 @@ static void launch_build(Engine& E, const PassBuf& B, bool cached, bool desc, ui
         const size_t lds = tab + waves * cache;
         const dim3 grid((slots + waves - 1) / waves), block(64 * waves);
         const uint32_t cs = max_half + 3;
-        if (desc)
+        if (stream64)
             hipLaunchKernelGGL((k_build_mix<K, true, uint64_t>), grid, block, lds, E.bstream, B.segs_d, slots, E.te0_d,
                                kRepl, cs, B.arena, B.walks, B.rnds);
         else
@@ -419,7 +429,7 @@ static void launch_build(Engine& E, const PassBuf& B, bool cached, bool desc, ui
     } else {
         const uint32_t waves = 4;
         const dim3 grid((slots + waves - 1) / waves), block(64 * waves);
-        if (desc)
+        if (stream64)
             hipLaunchKernelGGL((k_build_mix<K, false, uint64_t>), grid, block, tab, E.bstream, B.segs_d, slots, E.te0_d,
                                kRepl, 0u, B.arena, B.walks, B.rnds);
         else
@@ -436,10 +446,14 @@ static void launch_chain(Engine& E, const PassBuf& B, bool bgh, uint32_t lay) {
                        B.rnds, E.jc_d, B.pc_d, B.fail_d)
     if (bgh && lay == LAY_DESC)
         CH(true, LAY_DESC);
+    else if (bgh && lay == LAY_RES)
+        CH(true, LAY_RES);
     else if (bgh)
         CH(true, LAY_MAP);
     else if (lay == LAY_DESC)
         CH(false, LAY_DESC);
+    else if (lay == LAY_RES)
+        CH(false, LAY_RES);
     else
         CH(false, LAY_MAP);
 #undef CH
@@ -453,9 +467,9 @@ static int enqueue_pass(Engine& E, int bi) {
     EH(hipMemcpyAsync(B.segs_d, B.segs_h, size_t(B.nsegs) * sizeof(SegDev), hipMemcpyHostToDevice, E.bstream));
     EH(hipEventRecord(B.eb0, E.bstream));
     switch (lead.spec.p.k) {
-        case 2: launch_build<2>(E, B, lead.cached, lead.lay == LAY_DESC, max_half); break;
-        case 3: launch_build<3>(E, B, lead.cached, lead.lay == LAY_DESC, max_half); break;
-        default: launch_build<4>(E, B, lead.cached, lead.lay == LAY_DESC, max_half); break;
+        case 2: launch_build<2>(E, B, lead.cached, lead.lay != LAY_MAP, max_half); break;
+        case 3: launch_build<3>(E, B, lead.cached, lead.lay != LAY_MAP, max_half); break;
+        default: launch_build<4>(E, B, lead.cached, lead.lay != LAY_MAP, max_half); break;
     }
     EH(hipGetLastError());
     EH(hipEventRecord(B.eb1, E.bstream));
@@ -749,11 +763,12 @@ int ck_engine_submit(ck_engine* h, const ck_job* job, uint64_t* job_id) {
     j->S = make_shape(job->p);
     j->cached = job->p.half_bits <= kMaxCacheHalf &&
                 tab_bytes(kRepl) + (size_t(8) << job->p.half_bits) * 2 <= E.lds_block;
-    j->lay = job->p.k * j->S.dbits + 2 <= 63 ? LAY_DESC : LAY_MAP;
+    const uint32_t kb = job->p.k * j->S.dbits;
+    j->lay = kb + 2 <= 63 ? LAY_DESC : (kb - j->S.dbits + 2 <= 63 && kb <= 64) ? LAY_RES : LAY_MAP;
     j->bgh = job->p.mode == CK_MODE_BGH25;
     j->group = group_of(*j);
-    j->map_bytes = align256(size_t(job->p.pnum) * (j->lay == LAY_DESC ? 8 : size_t(job->p.k) * 4));
-    j->table_bytes = align256(size_t(job->p.buckets) * (j->lay == LAY_DESC ? 8 : 4));
+    j->map_bytes = align256(size_t(job->p.pnum) * (j->lay != LAY_MAP ? 8 : size_t(job->p.k) * 4));
+    j->table_bytes = align256(size_t(job->p.buckets) * (j->lay != LAY_MAP ? 8 : 4));
     if ((j->map_bytes + j->table_bytes) * SEG + 1024 > E.arena_per)
         return fail(CK_ENOMEM, "64 trials of this shape do not fit one pass buffer");
     j->bw = build_weight(job->p, j->cached);

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_digest_res(Shape S, uint32_t count, con
     for (uint32_t b = 0; b < S.buckets; ++b) {
         const E e = t[b];
         if (e == static_cast<E>(~E(0))) continue;
-        uint32_t c[K];
+        Cands c = {};
         const uint32_t choice = R.dec(e, b, c);
         uint64_t h = 0;
 #pragma unroll
@@ -1051,17 +1051,17 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
                    ck_stats* st) {
     c.budget = hbm_budget(c);
     // Auto: descriptor entries with load + store probes where they fit (fastest
-    // with tens of thousands of chains in flight); otherwise ordinal entries,
-    // with exchange probes when few chains are in flight (one round trip per
-    // probe matters when latency, not throughput, bounds the pass).
+    // with tens of thousands of chains in flight, and reusable without a clear);
+    // otherwise residual entries, still one random read per probe (measured
+    // against the bucket map at n = 40000, k = 4, m = 50000: 164K vs 141K
+    // trials/s; at n = 2^20, memory-capped: 2416 vs 2170); otherwise ordinal
+    // entries, with exchange probes when few chains are in flight (one round
+    // trip per probe matters when latency, not throughput, bounds the pass).
     uint32_t layout = c.layout;
     if (layout == CK_LAYOUT_AUTO) {
         if (!out.tables && desc_ok(p))
             layout = CK_LAYOUT_DESC_PLAIN;
-        else if (!out.tables && res_ok(p) && hbm_buffers(c, p, count, CK_LAYOUT_RESIDUAL) == 1)
-            // Memory-capped, single-buffered passes (n = 2^20): one random read
-            // per probe beats exchange probes plus a map read per eviction
-            // (measured 2416 vs 2170 trials/s).
+        else if (!out.tables && res_ok(p))
             layout = CK_LAYOUT_RESIDUAL;
         else
             layout = hbm_chunk(c, p, count, CK_LAYOUT_MAP_XCHG, hbm_buffers(c, p, count, CK_LAYOUT_MAP_XCHG)) < kXchgBelow
