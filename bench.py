@@ -195,7 +195,7 @@ def sweep_leg(ck, points: int, trials: int, plain_too: bool = True):
         plain_s = time.perf_counter() - t0
         ctx.close()
         print(f"sweep: per-point launches {plain_s:.2f} s", file=sys.stderr, flush=True)
-    eng = ck.Engine(0, policy=ck.SCHED_BALANCE)
+    eng = ck.Engine(0, policy=ck.SCHED_SRPT)
     t0 = time.perf_counter()
     ids = [eng.submit(p, pt["seed"], pt["trials"], bt=pt["bt"]) for pt, p in zip(pts, ps)]
     # Jobs finish in the scheduler's order, not in submission order: poll all

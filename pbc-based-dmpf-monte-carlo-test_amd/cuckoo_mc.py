@@ -256,7 +256,7 @@ class Engine:
     are the same trial-for-trial as Context.launch over the same ranges."""
 
     def __init__(self, device: int = 0, arena_bytes: int = 0, buffers: int = 0, max_slots: int = 0,
-                 max_jobs: int = 0, policy: int = SCHED_BALANCE, window: int = 0):
+                 max_jobs: int = 0, policy: int = SCHED_SRPT, window: int = 0):
         cfg = EngineConfig(arena_bytes, buffers, max_slots, max_jobs, policy, window, 0)
         h = C.c_void_p()
         _check(LIB.ck_engine_create(device, C.byref(cfg), C.byref(h)))

@@ -84,3 +84,20 @@ def test_tables_cleared_where_the_plan_changes():
             g, sm = reference(a, rk, 0, 2048)
             assert list(map(int, f)) == list(map(int, g)), tier
             assert st.probes == sm.probes, tier
+
+
+def test_graph_replays_across_tag_wraps():
+    """Graph mode rewrites each pass's clear flag and tag on every replay: with
+    2 tag bits a replay's passes alternate between clearing and reusing."""
+    p = ck.params("bgh25", 20000, 16, k=4, buckets=25000, kick=100, stash=1)
+    count = 3 * CHUNK + 100  # four passes: buffers 0, 1, 0, 1
+    with ck.Context(0) as g:
+        g.set_graph(True)
+        g.set_chunk(CHUNK)
+        for rep in range(5):
+            rk = ck.batch_key(21, rep)
+            f, st = g.launch(p, rk, rep * 5000, count)
+            assert st.graph == (2 if rep == 0 else 1)
+            r, sr = reference(p, rk, rep * 5000, count)
+            assert list(map(int, f)) == list(map(int, r)), rep
+            assert st.probes == sr.probes, rep
