@@ -1384,7 +1384,12 @@ static int launch_common(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44],
         const uint32_t tier = choose_tier(ctx->c, *p);
         rc = tier == CK_TIER_LDS ? run_lds(ctx->c, *p, rk, first, count, out, stats)
                                  : run_hbm(ctx->c, *p, rk, first, count, out, stats);
-        if (rc) return rc;
+        if (rc) {
+            // A failed call may have planned clears that never ran: the next
+            // call starts from cleared tables.
+            forget_tables(ctx->c);
+            return rc;
+        }
     }
     std::sort(fails.begin(), fails.end());
     if (std::adjacent_find(fails.begin(), fails.end()) != fails.end())
@@ -1496,6 +1501,7 @@ int ck_trial_map(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_
     ctx->c.chunk_cap = 1;
     rc = run_hbm(ctx->c, *p, rk, offset, 1, out, nullptr);
     ctx->c.chunk_cap = saved;
+    if (rc) forget_tables(ctx->c);
     return rc;
 }
 
