@@ -3,6 +3,8 @@
 Bit-exact: failure offsets, per-trial probe counts, final tables, the device
 key derivation and bucket map.  Sizes are chosen so the oracle finishes in
 seconds."""
+import hashlib
+
 import numpy as np
 import pytest
 
@@ -89,6 +91,37 @@ def test_reference_ranges(ctx, tier, fixtures):
         p = ck.params(row["mode"], row["pnum"], row["ell_i"], input_mode=row["input_mode"])
         f, _ = ctx.launch(p, ck.batch_key(row["seed"], row["batch"]), 0, row["count"])
         assert list(f) == row["failures"], row
+
+
+def test_reference_trial_tables(ctx, tier, fixtures):
+    """Outcome and final table of single trials as the reference's
+    cpu_trial_cached leaves them (cpu_trials.hpp:128-149), with its own
+    mode-derived buckets and kick (3n for bgh25: 49152 at n = 2^14).  The
+    fixture holds sha256 over the reference's u32 entries ordinal | choice<<16."""
+    for row in fixtures["reference_trials"]:
+        p = ck.params(row["mode"], row["pnum"], row["ell_i"], input_mode=row["input_mode"])
+        f, _, tables, _ = ctx.launch_debug(p, ck.batch_key(row["seed"], row["batch"]), row["offset"], 1,
+                                           tables=True)
+        assert len(f) == row["failed"], row
+        t = tables[0].copy()
+        occ = t != 0xFFFFFFFF
+        t[occ] = (t[occ] & 0x3FFFFFFF) | ((t[occ] >> 30) << 16)
+        assert hashlib.sha256(t.astype("<u4").tobytes()).hexdigest() == row["table_sha256"], row
+
+
+def test_headline_shape_reference_kick(ctx):
+    """The headline shape with the reference's own kick bound (3n = 49152)
+    instead of 500: no insertion there nears 500 kicks, so failures and
+    probe counts are those of the 500-kick run, and both equal the oracle."""
+    p_ref, op_ref = both("bgh25", 16384, 16)
+    p_500 = ck.params("bgh25", 16384, 16, kick=500)
+    assert p_ref.kick == 49152 and p_ref.buckets == 24576
+    count = 2048
+    f, probes, _, _ = ctx.launch_debug(p_ref, ck.batch_key(1, 0), 0, count)
+    g, probes500, _, _ = ctx.launch_debug(p_500, ck.batch_key(1, 0), 0, count)
+    of, oprobes, _ = ock.run_range(ock.batch_key(1, 0), 0, count, op_ref, probes=True)
+    assert list(f) == list(of) == list(g)
+    assert (probes == oprobes).all() and (probes == probes500).all()
 
 
 PROBE_SHAPES = [
