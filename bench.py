@@ -338,27 +338,31 @@ def graph_leg(ck, p, steps: int, trials: int):
 def hbm_leg(ck, trials: int, layout: int = 0, warm: int = 0):
     """Rare-event HBM shape: n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
     ell_i 20, seed 4 -- the first `trials` trials of batch 0, after an untimed
-    launch of `warm` trials of batch 1 (default: as many, so that the
-    workspace, 95 % of the free memory, is allocated outside the timed region;
-    a 64-trial warm-up allocates a small one)."""
+    launch of `warm` trials of batch 1 (default: 2^15, more than one pass
+    holds, so that the workspace, 95 % of the free memory, is allocated outside
+    the timed region).  trials = 0 times two full passes of the size that
+    launch used."""
     p = ck.params("bgh25", 1 << 20, 20, k=3, buckets=1363149, kick=1000, stash=2)
     ctx = ck.Context(0)
     ctx.set_layout(layout)
-    ctx.launch(p, ck.batch_key(4, 1), 0, warm or trials)
+    _, ws = ctx.launch(p, ck.batch_key(4, 1), 0, warm or max(trials, 1 << 15))
+    trials = trials or 2 * ws.chunk
     t0 = time.perf_counter()
     f, st = ctx.launch(p, ck.batch_key(4, 0), 0, trials)
     dt = time.perf_counter() - t0
     ctx.close()
     chain_s = st.chain_ms / 1e3
-    passes = max(1, st.launches // 2)
-    pass_s = (st.build_ms + st.chain_ms) / 1e3 / passes  # single-buffered: a pass is its build, then its chain
+    passes = -(-trials // st.chunk)
     return {
         "workload": f"bgh25 n=2^20 m=1363149 k=3 stash=2 kick=1000 ell_i=20 seed=4, trials [0, {trials}) of batch 0",
         "trials_per_s": trials / dt, "probes_per_s": st.probes / dt, "probes_per_trial": st.probes / trials,
         "failed": len(f), "trials_per_pass": st.chunk, "passes": passes,
+        # Each pass inserts its trials' items in windows (build + chain launches)
+        # when memory caps the pass; launches / passes / 2 is the window count.
+        "windows_per_pass": st.launches // (2 * passes),
         # Chains are latency-bound, so a pass takes about as long whatever its trial
         # count: a long run of full passes goes at trials_per_pass / pass time.
-        "full_pass_trials_per_s": st.chunk / pass_s if pass_s > 0 else None,
+        "full_pass_trials_per_s": st.chunk * passes / dt,
         "chain_algorithmic_gbs": ((PROBE_BYTES * st.probes + ITEM_BYTES * p.pnum * trials) / chain_s / 1e9
                                   if chain_s > 0 else None),
         "device_ms": {"build": st.build_ms, "chain": st.chain_ms},
@@ -386,8 +390,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=100)
     ap.add_argument("--serve-budget", default="10,1e5", help="trial budget range lo,hi (log-uniform)")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--hbm-trials", type=int, default=28672,
-                    help="trials of the n=2^20 leg (default: two full memory-capped passes)")
+    ap.add_argument("--hbm-trials", type=int, default=0,
+                    help="trials of the n=2^20 leg (default 0: two full memory-capped passes)")
     ap.add_argument("--no-hbm", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-mode comparison")
@@ -413,7 +417,7 @@ def main():
               "desc-plain": ck.LAYOUT_DESC_PLAIN, "residual": ck.LAYOUT_RESIDUAL}[args.layout]
     if args.hbm_only:
         # Counter profiles: a 64-trial warm-up keeps the dispatches to profile few.
-        print(json.dumps({"hbm_tier": hbm_leg(ck, args.hbm_trials, layout, warm=64)}), flush=True)
+        print(json.dumps({"hbm_tier": hbm_leg(ck, args.hbm_trials or (1 << 15), layout, warm=64)}), flush=True)
         return 0
     p = ck.params(SHAPE["mode"], SHAPE["pnum"], SHAPE["ell"], kick=SHAPE["kick"])
     ctx = ck.Context(0)

@@ -98,6 +98,14 @@ void ck_ctx_destroy(ck_ctx* ctx);
 int ck_ctx_set_tier(ck_ctx* ctx, uint32_t tier);
 /* Caps the trials per device pass (0 = library default). */
 int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
+/* Residual-layout item windows: each HBM-tier pass inserts the items in
+ * windows of `items` (0 = automatic: windows of pnum/32 when memory caps a pass
+ * below its target, else none).  A window's build writes the candidates of its
+ * items only, so a slot needs less memory and more trials fit in a pass.
+ * Results do not depend on it.  Not applied in graph mode or to launches that
+ * return the bucket map, walks or random words.  No reference counterpart (the
+ * reference keeps no per-trial candidate stream). */
+int ck_ctx_set_window(ck_ctx* ctx, uint32_t items);
 /* Forces an HBM-tier table layout (CK_LAYOUT_*). */
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
 /* Graph mode (on = 1): HBM-tier ck_launch calls run as one hipGraph of all

@@ -359,17 +359,26 @@ struct CandStream {
     }
 };
 
+// Items [begin, end) that one residual-layout launch inserts; the candidate
+// stream holds them from index 0.  A trial may run as several such windows in
+// order: each ends at an item boundary, where nothing is in hand and the whole
+// chain state is the walk, the random word, the stash count and the probe count.
+struct ItemSpan {
+    uint32_t begin, end;
+};
+
 // BGH25 over the residual layout (same decisions as chain_bgh).
 template <int K, class E, class M>
 __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
-                                              E* __restrict__ t, uint64_t walk, uint64_t rnd, uint32_t& probes) {
+                                              E* __restrict__ t, ItemSpan span, uint64_t& walk, uint64_t& rnd,
+                                              uint32_t& stash_left, uint32_t& probes) {
     constexpr E kEmpty = static_cast<E>(~E(0));
-    uint32_t item = 0, coll = 0, stash_left = S.stash, pr = 0;
+    uint32_t item = span.begin, coll = 0, pr = 0;
     Cands c = {}, nx = {};
     st.load(0, c);
-    if (S.pnum > 1) st.load(1, nx);
+    if (item + 1 < span.end) st.load(1, nx);
     bool failed = false;
-    while (item < S.pnum) {
+    while (item < span.end) {
         if (rnd < K) rnd = splitmix(walk);
         const uint32_t which = take_choice<K>(rnd);
         const uint32_t b = pick<K>(c, which);
@@ -389,7 +398,7 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
         if (advance) {
             ++item;
             c = nx;
-            if (item + 1 < S.pnum) st.load(item + 1, nx);
+            if (item + 1 < span.end) st.load(item + 1 - span.begin, nx);
             coll = 0;
         } else {
             R.dec(old, b, c);
@@ -402,15 +411,16 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
 // DRRT18 over the residual layout (same decisions as chain_drrt).
 template <int K, class E, class M>
 __device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
-                                               E* __restrict__ t, uint32_t& probes) {
+                                               E* __restrict__ t, ItemSpan span, uint32_t& stash_left,
+                                               uint32_t& probes) {
     constexpr E kEmpty = static_cast<E>(~E(0));
-    uint32_t pr = 0, stash_left = S.stash;
+    uint32_t pr = 0;
     bool failed = false;
     Cands nx = {};
     st.load(0, nx);
-    for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
+    for (uint32_t item = span.begin; item < span.end && !failed; ++item) {
         Cands c = nx;
-        if (item + 1 < S.pnum) st.load(item + 1, nx);
+        if (item + 1 < span.end) st.load(item + 1 - span.begin, nx);
         bool placed = false;
 #pragma unroll
         for (int w = 0; w < K; ++w) {

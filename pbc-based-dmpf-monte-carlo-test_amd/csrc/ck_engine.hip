@@ -145,10 +145,13 @@ __global__ __launch_bounds__(256) void k_chain_mix(const SegDev* __restrict__ se
             uint64_t* t = reinterpret_cast<uint64_t*>(arena + G.table_off + size_t(lane) * G.table_bytes);
             const ResCodec<K, uint64_t> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
             const CandStream<K, uint64_t> st{m, S.dbits, (1ull << S.dbits) - 1};
-            if constexpr (BGH)
-                failed = chain_bgh_res<K>(S, R, st, t, walks[slot], rnds[slot], pr);
-            else
-                failed = chain_drrt_res<K>(S, R, st, t, pr);
+            uint32_t stash_left = S.stash;
+            if constexpr (BGH) {
+                uint64_t walk = walks[slot], rnd = rnds[slot];
+                failed = chain_bgh_res<K>(S, R, st, t, ItemSpan{0, S.pnum}, walk, rnd, stash_left, pr);
+            } else {
+                failed = chain_drrt_res<K>(S, R, st, t, ItemSpan{0, S.pnum}, stash_left, pr);
+            }
         } else {
             const uint32_t* m = reinterpret_cast<const uint32_t*>(arena + G.map_off + size_t(lane) * G.map_bytes);
             uint32_t* t = reinterpret_cast<uint32_t*>(arena + G.table_off + size_t(lane) * G.table_bytes);

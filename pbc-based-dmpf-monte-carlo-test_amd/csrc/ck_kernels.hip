@@ -53,13 +53,15 @@ __device__ __forceinline__ LdsTab stage_tab(uint32_t* lds, uint32_t shift) {
 // descriptor layout (one packed candidate list per ordinal).  With `clear`, the
 // slot's table (table_bytes, a multiple of 16) is filled with all-ones (empty);
 // without, the chain's pass tag makes the previous passes' entries read as
-// empty.
+// empty.  The map rows cover ordinals [item0, item1), row 0 being item0; the
+// walk and random word are written with the first window (item0 = 0).
 template <int K, bool CACHED, class M>
 __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
                                                uint32_t tab_shift, M* __restrict__ map, size_t map_stride,
                                                char* __restrict__ tables, size_t table_bytes,
                                                uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
-                                               uint32_t* __restrict__ key_out, uint32_t clear) {
+                                               uint32_t* __restrict__ key_out, uint32_t clear, uint32_t item0,
+                                               uint32_t item1) {
     extern __shared__ uint32_t lds[];
     const LdsTab T = stage_tab(lds, tab_shift);
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -78,22 +80,22 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys
     if constexpr (CACHED) {
         uint16_t* cache = reinterpret_cast<uint16_t*>(lds + (256u << tab_shift)) + (size_t(wave) << (S.half + 3));
         build_cache(T, ts, S, cache, lane);
-        for (uint32_t o = lane; o < S.pnum; o += 64) {
+        for (uint32_t o = item0 + lane; o < item1; o += 64) {
             uint32_t c[K];
             feistel_cached_k<K>(cache, S, element_of(S, ts.shift, ts.step, o), c);
-            store_candidates<K>(m, S, o, c);
+            store_candidates<K>(m, S, o - item0, c);
         }
     } else {
-        for (uint32_t o = lane; o < S.pnum; o += 64) {
+        for (uint32_t o = item0 + lane; o < item1; o += 64) {
             const uint64_t el = element_of(S, ts.shift, ts.step, o);
             uint32_t c[K];
             for (int w = 0; w < K; ++w) c[w] = feistel_direct(T, ts.rk, S, el, K, w);
-            store_candidates<K>(m, S, o, c);
+            store_candidates<K>(m, S, o - item0, c);
         }
     }
 
     if (clear) clear_table(tables + slot * table_bytes, table_bytes, lane);
-    if (lane == 0) {
+    if (lane == 0 && item0 == 0) {
         walks[slot] = ts.walk;
         rnds[slot] = ts.rnd;
     }
@@ -181,27 +183,48 @@ __global__ __launch_bounds__(256) void k_digest(Shape S, uint32_t count, const M
 }
 
 // chain (HBM tier, residual layout): M is the candidate stream (u64 packed
-// descriptor or u32 map rows), E the residual table entry (u32 or u64).
+// descriptor or u32 map rows), E the residual table entry (u32 or u64).  The
+// launch inserts items [item0, item1), whose candidates the stream holds.
+// With `state` (a trial split into windows of items), the chain resumes from
+// and saves, per slot: the walk and random word, state = stash left | failed
+// << 31, and probes_out = probes so far; a failed trial is skipped by later
+// windows.  Without it, item0 = 0 and item1 = pnum.
 template <int K, bool BGH, class M, class E>
 __global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint32_t count,
                                                    const M* __restrict__ map, size_t map_stride,
                                                    char* __restrict__ tables, size_t table_bytes,
-                                                   const uint64_t* __restrict__ walks,
-                                                   const uint64_t* __restrict__ rnds, uint32_t* __restrict__ counters,
-                                                   uint32_t* __restrict__ fail_offsets,
-                                                   uint32_t* __restrict__ probes_out) {
+                                                   uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
+                                                   uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
+                                                   uint32_t* __restrict__ probes_out, uint32_t item0, uint32_t item1,
+                                                   uint32_t* __restrict__ state) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     bool failed = false;
     uint32_t pr = 0;
     if (slot < count) {
-        const ResCodec<K, E> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
-        const CandStream<K, M> st{map + slot * map_stride, S.dbits, (1ull << S.dbits) - 1};
-        E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
-        if constexpr (BGH)
-            failed = chain_bgh_res<K>(S, R, st, t, walks[slot], rnds[slot], pr);
-        else
-            failed = chain_drrt_res<K>(S, R, st, t, pr);
-        if (probes_out) probes_out[slot] = pr;
+        uint32_t flags = S.stash, before = 0;
+        if (state && item0) {
+            flags = state[slot];
+            before = probes_out[slot];
+        }
+        if (!(flags >> 31)) {
+            const ResCodec<K, E> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+            const CandStream<K, M> st{map + slot * map_stride, S.dbits, (1ull << S.dbits) - 1};
+            E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
+            const ItemSpan span{item0, item1};
+            uint32_t stash_left = flags;
+            if constexpr (BGH) {
+                uint64_t walk = walks[slot], rnd = rnds[slot];
+                failed = chain_bgh_res<K>(S, R, st, t, span, walk, rnd, stash_left, pr);
+                if (state) {
+                    walks[slot] = walk;
+                    rnds[slot] = rnd;
+                }
+            } else {
+                failed = chain_drrt_res<K>(S, R, st, t, span, stash_left, pr);
+            }
+            if (state) state[slot] = stash_left | (failed ? 1u << 31 : 0u);
+        }
+        if (probes_out) probes_out[slot] = before + pr;
     }
     append_failures(failed, first + slot, counters + CNT_FAIL, fail_offsets);
     add_wave_total(pr, reinterpret_cast<unsigned long long*>(counters + CNT_PROBES));
@@ -417,6 +440,7 @@ struct Ctx {
     uint32_t tier = CK_TIER_AUTO;
     uint32_t layout = CK_LAYOUT_AUTO;
     uint32_t chunk_cap = 0;
+    uint32_t window = 0;  // items per residual-layout window (0 = automatic)
     size_t budget = size_t(96) << 30;  // HBM budget for the double-buffered maps and tables (set per call)
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
@@ -529,20 +553,28 @@ static bool res_ok(const ck_params& p) { return res_fits(p, 64); }
 static bool res_e32(const ck_params& p) { return res_fits(p, 32); }
 static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits <= 64; }
 
-// HBM-tier pass buffer: [counters | map C x map_bytes | tables C x table_bytes |
-// walk C | rnd C | probes C | fail C | keys C x 44 | digests C], twice
-// (double-buffered).  In the descriptor layout the "map" is one u64 candidate
-// descriptor per ordinal and the table entries are u64.
+// HBM-tier pass buffer: [counters | map C x map_bytes (x2 with windows) |
+// tables C x table_bytes | walk C | rnd C | probes C | fail C | keys C x 44 |
+// digests C | state C], twice (double-buffered).  In the descriptor layout the
+// "map" is one u64 candidate descriptor per ordinal and the table entries are
+// u64.  With item windows (residual layout) the map holds the candidates of
+// `window` items, in two halves: the build of window j+1 fills one while the
+// chain of window j reads the other.
 struct HbmPlan {
     bool desc, map16, table16, xchg;
     bool res, stream64, e32;  // residual layout: u64 candidate stream, u32 entries
+    uint32_t window;     // items per map (pnum unless windowed)
+    uint32_t nwin;       // windows per trial
     size_t map_stride;   // map entries per slot
     size_t map_bytes;    // per slot
     size_t table_bytes;  // per slot
-    size_t off_cnt, off_map, off_tables, off_walk, off_rnd, off_probes, off_fail, off_key, off_digest, buffer;
+    size_t off_cnt, off_map, off_tables, off_walk, off_rnd, off_probes, off_fail, off_key, off_digest, off_state,
+        buffer;
 };
-static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout) {
+static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uint32_t window) {
     HbmPlan h{};
+    h.window = window;
+    h.nwin = (p.pnum + window - 1) / window;
     const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     h.desc = desc;
     h.xchg = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_MAP_XCHG;
@@ -554,14 +586,14 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout) {
     const bool m64 = desc || h.stream64;
     const size_t mw = m64 ? 8 : h.map16 ? 2 : 4;
     const size_t tw = h.res ? (h.e32 ? 4 : 8) : desc ? 8 : h.table16 ? 2 : 4;
-    h.map_bytes = align256(size_t(p.pnum) * (m64 ? 1 : p.k) * mw);
+    h.map_bytes = align256(size_t(window) * (m64 ? 1 : p.k) * mw);
     h.map_stride = h.map_bytes / mw;
     h.table_bytes = align256(size_t(p.buckets) * tw);
     size_t o = 0;
     h.off_cnt = o;
     o = align256(o + CNT_WORDS * 4);
     h.off_map = o;
-    o = align256(o + h.map_bytes * chunk);
+    o = align256(o + h.map_bytes * chunk * (h.nwin > 1 ? 2 : 1));
     h.off_tables = o;
     o = align256(o + h.table_bytes * chunk);
     h.off_walk = o;
@@ -576,6 +608,8 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout) {
     o = align256(o + size_t(chunk) * 44 * 4);
     h.off_digest = o;
     o = align256(o + size_t(chunk) * 8);
+    h.off_state = o;
+    o = align256(o + (h.nwin > 1 ? size_t(chunk) * 4 : 0));
     h.buffer = o;
     return h;
 }
@@ -587,9 +621,9 @@ static constexpr uint32_t kXchgBelow = 1u << 14;
 // Trials per pass: 65536 chains in flight saturate the random-access path
 // (measured: 32768 slower, 131072 and more no faster).
 static constexpr uint32_t kHbmChunk = 1u << 16;
-static size_t slot_bytes(const ck_params& p, uint32_t layout) {
-    const HbmPlan one = plan_hbm(p, 1, layout);
-    return one.map_bytes + one.table_bytes + 8 + 8 + 4 + 4 + 176;
+static size_t slot_bytes(const ck_params& p, uint32_t layout, uint32_t window) {
+    const HbmPlan one = plan_hbm(p, 1, layout, window);
+    return one.map_bytes * (one.nwin > 1 ? 2 : 1) + one.table_bytes + 8 + 8 + 4 + 4 + 176 + (one.nwin > 1 ? 4 : 0);
 }
 // Pass buffers: two, so the build of pass i+1 runs under the chain of pass i,
 // unless memory caps a pass below its target.  Then one buffer of twice the
@@ -597,12 +631,32 @@ static size_t slot_bytes(const ck_params& p, uint32_t layout) {
 // latency of its chains, and its build is a few percent of the chain time.
 static uint32_t hbm_buffers(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout) {
     const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
-    return c.budget / (2 * slot_bytes(p, layout)) < std::min<size_t>(target, count) ? 1 : 2;
+    return c.budget / (2 * slot_bytes(p, layout, p.pnum)) < std::min<size_t>(target, count) ? 1 : 2;
 }
-static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout, uint32_t nbuf) {
-    const size_t fit = std::max<size_t>(1, c.budget / (nbuf * slot_bytes(p, layout)));
+static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout, uint32_t nbuf,
+                          uint32_t window) {
+    const size_t fit = std::max<size_t>(1, c.budget / (nbuf * slot_bytes(p, layout, window)));
     const uint32_t target = c.chunk_cap ? c.chunk_cap : kHbmChunk;
     return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>({fit, size_t(target), size_t(count)})));
+}
+
+// Item windows (residual layout).  When memory caps a single-buffered pass
+// below its target, each slot keeps the candidates of 2 x pnum / kWindows items
+// instead of pnum, and a pass runs as kWindows build + chain launches, each
+// chain stopping at an item boundary.  The candidate stream is 44 % of a slot
+// at n = 2^20 (8 of 19.3 MB); the memory that windows free holds more slots,
+// and these chains are latency-bound: a pass's chain time stays at 4.1 s from
+// 15,137 to 25,552 chains.  Measured at n = 2^20 (trials/s): no windows 3,425;
+// 8 windows 5,346; 16: 5,813; 32: 6,089.  Each window's build derives the
+// trial and its round cache again, under the previous window's chain.
+static constexpr uint32_t kWindows = 32;
+static uint32_t stream_window(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout, bool allowed) {
+    if (!allowed || layout != CK_LAYOUT_RESIDUAL) return p.pnum;
+    if (c.window) return std::min(c.window, p.pnum);
+    if (hbm_buffers(c, p, count, layout) != 1) return p.pnum;
+    const uint32_t target = std::min(c.chunk_cap ? c.chunk_cap : kHbmChunk, count);
+    if (hbm_chunk(c, p, count, layout, 1, p.pnum) >= target) return p.pnum;
+    return (p.pnum + kWindows - 1) / kWindows;
 }
 
 // HBM budget of the pass buffers: 95 % of the free device memory (the current
@@ -675,6 +729,8 @@ struct PassArgs {
     unsigned long long* digests;  // null unless requested
     uint32_t clear;    // the build clears the tables
     uint64_t tag;      // descriptor-entry tag of the pass
+    uint32_t item0, item1;  // items whose candidates the map holds (residual windows; else 0, pnum)
+    uint32_t* state;   // residual windows: per-slot chain state; else null
 };
 
 template <int K, class M>
@@ -683,10 +739,10 @@ static void launch_build_t(const BuildPlan& bp, const PassArgs& a, Dispatch& st)
     M* map = reinterpret_cast<M*>(a.map);
     if (bp.cached)
         st(&k_build<K, true, M>, dim3(blocks), dim3(64 * bp.waves), bp.lds, a.S, a.bk, a.first, a.count, kRepl, map,
-           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys, a.clear);
+           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys, a.clear, a.item0, a.item1);
     else
         st(&k_build<K, false, M>, dim3(blocks), dim3(64 * bp.waves), bp.lds, a.S, a.bk, a.first, a.count, kRepl, map,
-           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys, a.clear);
+           a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.keys, a.clear, a.item0, a.item1);
 }
 template <int K>
 static void launch_build_k(const BuildPlan& bp, const HbmPlan& h, const PassArgs& a, Dispatch& st) {
@@ -735,7 +791,7 @@ static void launch_chain_res_t(const PassArgs& a, Dispatch& st) {
     const uint32_t blocks = (a.count + 255) / 256;
     st(&k_chain_res<K, BGH, M, E>, dim3(blocks), dim3(256), 0, a.S, a.first, a.count,
        reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.cnt, a.fail,
-       a.probes);
+       a.probes, a.item0, a.item1, a.state);
 }
 template <int K, bool BGH>
 static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
@@ -1064,7 +1120,8 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         else if (!out.tables && res_ok(p))
             layout = CK_LAYOUT_RESIDUAL;
         else
-            layout = hbm_chunk(c, p, count, CK_LAYOUT_MAP_XCHG, hbm_buffers(c, p, count, CK_LAYOUT_MAP_XCHG)) < kXchgBelow
+            layout = hbm_chunk(c, p, count, CK_LAYOUT_MAP_XCHG, hbm_buffers(c, p, count, CK_LAYOUT_MAP_XCHG),
+                               p.pnum) < kXchgBelow
                          ? CK_LAYOUT_MAP_XCHG
                          : CK_LAYOUT_MAP;
     }
@@ -1075,9 +1132,12 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         return fail(CK_EINVAL, "residual layout needs (k-1)*bits(buckets-1) + 2 <= 63");
     if (layout == CK_LAYOUT_RESIDUAL && out.tables)
         return fail(CK_EINVAL, "the residual layout does not keep ordinals; use the map layout for tables");
-    const uint32_t nbuf = hbm_buffers(c, p, count, layout);
-    const uint32_t chunk = hbm_chunk(c, p, count, layout, nbuf);
-    const HbmPlan h = plan_hbm(p, chunk, layout);
+    const bool graph =
+        c.graph_on && !out.probes && !out.tables && !out.keys && !out.walks && !out.rnds && !out.map && !out.digests;
+    const uint32_t window = stream_window(c, p, count, layout, !graph && !out.map && !out.walks && !out.rnds);
+    const uint32_t nbuf = window < p.pnum ? 1 : hbm_buffers(c, p, count, layout);
+    const uint32_t chunk = hbm_chunk(c, p, count, layout, nbuf, window);
+    const HbmPlan h = plan_hbm(p, chunk, layout, window);
     int rc = ensure_ws(c, nbuf * h.buffer);
     if (rc) return rc;
     const BuildPlan bp = plan_build(p, c.lds_block);
@@ -1096,9 +1156,12 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         a.rnds = reinterpret_cast<uint64_t*>(base + h.off_rnd);
         a.cnt = reinterpret_cast<uint32_t*>(base + h.off_cnt);
         a.fail = reinterpret_cast<uint32_t*>(base + h.off_fail);
-        a.probes = out.probes ? reinterpret_cast<uint32_t*>(base + h.off_probes) : nullptr;
+        a.probes = out.probes || h.nwin > 1 ? reinterpret_cast<uint32_t*>(base + h.off_probes) : nullptr;
         a.keys = out.keys ? reinterpret_cast<uint32_t*>(base + h.off_key) : nullptr;
         a.digests = out.digests ? reinterpret_cast<unsigned long long*>(base + h.off_digest) : nullptr;
+        a.item0 = 0;
+        a.item1 = p.pnum;
+        a.state = h.nwin > 1 ? reinterpret_cast<uint32_t*>(base + h.off_state) : nullptr;
     }
     if (st) {
         st->tier = CK_TIER_HBM;
@@ -1112,7 +1175,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         forget_tables(c);
         c.table_plan = plan;
     }
-    if (c.graph_on && !out.probes && !out.tables && !out.keys && !out.walks && !out.rnds && !out.map && !out.digests)
+    if (graph)
         return run_hbm_graph(c, p, first, count, layout, nbuf, chunk, h, bp, args, taggable, tag_bits, out, st);
     auto enqueue = [&](uint32_t i) -> int {
         const int b = nbuf == 2 ? int(i & 1) : 0;
@@ -1147,7 +1210,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         }
         return CK_OK;
     };
-    auto gather = [&](uint32_t i) -> int {
+    auto gather = [&](uint32_t i, bool timed) -> int {
         const int b = nbuf == 2 ? int(i & 1) : 0;
         const PassArgs& a = args[b];
         CKH(hipEventSynchronize(c.ev_chain[b][1]));
@@ -1178,7 +1241,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         if (out.keys) CKH(hipMemcpy(out.keys + done * 44, a.keys, size_t(a.count) * 44 * 4, hipMemcpyDeviceToHost));
         if (out.walks) CKH(hipMemcpy(out.walks + done, a.walks, size_t(a.count) * 8, hipMemcpyDeviceToHost));
         if (out.rnds) CKH(hipMemcpy(out.rnds + done, a.rnds, size_t(a.count) * 8, hipMemcpyDeviceToHost));
-        if (st) {
+        if (st && timed) {
             float bms = 0, cms = 0;
             CKH(hipEventElapsedTime(&bms, c.ev_build[b][0], c.ev_build[b][1]));
             CKH(hipEventElapsedTime(&cms, c.ev_chain[b][0], c.ev_chain[b][1]));
@@ -1188,17 +1251,85 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         }
         return CK_OK;
     };
+    // Item windows: pass i runs as h.nwin (build, chain) pairs over one buffer.
+    // The build of window j fills map half j & 1 on the build stream while the
+    // chain of window j - 1 reads the other half; window j's build is enqueued
+    // once the chain of window j - 2, the last reader of its half, has ended.
+    // Only window 0 clears the tables and writes the walks.  Window parity e
+    // uses the event pairs ev_build[e] and ev_chain[e].
+    auto windowed = [&](uint32_t i) -> int {
+        PassArgs& a = args[0];
+        a.first = first + i * chunk;
+        a.count = std::min(chunk, count - i * chunk);
+        pass_tags(c, 0, false, 0, a);
+        auto account = [&](uint32_t j) -> int {
+            const int e = int(j & 1);
+            CKH(hipEventSynchronize(c.ev_chain[e][1]));
+            if (st) {
+                float bms = 0, cms = 0;
+                CKH(hipEventElapsedTime(&bms, c.ev_build[e][0], c.ev_build[e][1]));
+                CKH(hipEventElapsedTime(&cms, c.ev_chain[e][0], c.ev_chain[e][1]));
+                st->build_ms += bms;
+                st->chain_ms += cms;
+                st->launches += 2;
+            }
+            return CK_OK;
+        };
+        for (uint32_t j = 0; j < h.nwin; ++j) {
+            const int e = int(j & 1);
+            if (j >= 2 && (rc = account(j - 2))) return rc;
+            PassArgs w = a;
+            w.map = a.map + size_t(e) * h.map_bytes * chunk;
+            w.item0 = j * h.window;
+            w.item1 = std::min(p.pnum, w.item0 + h.window);
+            w.clear = j == 0 ? 1u : 0u;
+            CKH(hipEventRecord(c.ev_build[e][0], c.build_stream));
+            Dispatch bd, cd;
+            bd.stream = c.build_stream;
+            cd.stream = c.stream;
+            launch_build(p.k, bp, h, w, bd);
+            CKH(bd.err);
+            CKH(hipGetLastError());
+            CKH(hipEventRecord(c.ev_build[e][1], c.build_stream));
+            CKH(hipStreamWaitEvent(c.stream, c.ev_build[e][1], 0));
+            if (j == 0) cd(&k_zero, dim3(1), dim3(64), 0, a.cnt, CNT_WORDS);
+            CKH(hipEventRecord(c.ev_chain[e][0], c.stream));
+            launch_chain(p.k, bgh, h, w, cd);
+            CKH(cd.err);
+            CKH(hipGetLastError());
+            CKH(hipEventRecord(c.ev_chain[e][1], c.stream));
+        }
+        for (uint32_t j = h.nwin - std::min<uint32_t>(h.nwin, 2); j < h.nwin; ++j)
+            if ((rc = account(j))) return rc;
+        if (a.digests) {
+            Dispatch cd;
+            cd.stream = c.stream;
+            switch (p.k) {
+                case 2: launch_digest_k<2>(h, a, cd); break;
+                case 3: launch_digest_k<3>(h, a, cd); break;
+                default: launch_digest_k<4>(h, a, cd); break;
+            }
+            CKH(cd.err);
+            CKH(hipGetLastError());
+        }
+        return gather(i, false);
+    };
+    if (h.nwin > 1) {
+        for (uint32_t i = 0; i < passes; ++i)
+            if ((rc = windowed(i))) return rc;
+        return CK_OK;
+    }
     if (nbuf == 1) {
         for (uint32_t i = 0; i < passes; ++i)
-            if ((rc = enqueue(i)) || (rc = gather(i))) return rc;
+            if ((rc = enqueue(i)) || (rc = gather(i, true))) return rc;
         return CK_OK;
     }
     if ((rc = enqueue(0))) return rc;
     for (uint32_t i = 1; i < passes; ++i) {
         if ((rc = enqueue(i))) return rc;
-        if ((rc = gather(i - 1))) return rc;
+        if ((rc = gather(i - 1, true))) return rc;
     }
-    return gather(passes - 1);
+    return gather(passes - 1, true);
 }
 
 // LDS tier: one fused persistent launch per chunk.
@@ -1346,6 +1477,12 @@ int ck_ctx_set_graph(ck_ctx* ctx, uint32_t on) {
 int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials) {
     if (!ctx) return fail(CK_EINVAL, "null ctx");
     ctx->c.chunk_cap = trials;
+    return CK_OK;
+}
+
+int ck_ctx_set_window(ck_ctx* ctx, uint32_t items) {
+    if (!ctx) return fail(CK_EINVAL, "null ctx");
+    ctx->c.window = items;
     return CK_OK;
 }
 

@@ -92,6 +92,7 @@ def _load():
         "ck_ctx_destroy": (None, [vp]),
         "ck_ctx_set_tier": (C.c_int, [vp, U32]),
         "ck_ctx_set_chunk": (C.c_int, [vp, U32]),
+        "ck_ctx_set_window": (C.c_int, [vp, U32]),
         "ck_ctx_set_layout": (C.c_int, [vp, U32]),
         "ck_ctx_set_graph": (C.c_int, [vp, U32]),
         "ck_launch_digest": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U64),
@@ -123,7 +124,8 @@ LIB = _load()
 
 # Every symbol declared in include/cuckoo_mc.h.
 EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "ck_ctx_destroy",
-           "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_ctx_set_layout", "ck_ctx_set_graph", "ck_aes_self_test", "ck_launch",
+           "ck_ctx_set_tier", "ck_ctx_set_chunk", "ck_ctx_set_window", "ck_ctx_set_layout", "ck_ctx_set_graph",
+           "ck_aes_self_test", "ck_launch",
            "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_ctx_tune", "ck_engine_create",
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
            "ck_engine_release", "ck_engine_stats_get"]
@@ -188,6 +190,10 @@ class Context:
 
     def set_chunk(self, trials: int):
         _check(LIB.ck_ctx_set_chunk(self.h, trials))
+
+    def set_window(self, items: int):
+        """Residual-layout item window (0 = automatic)."""
+        _check(LIB.ck_ctx_set_window(self.h, items))
 
     def set_layout(self, layout: int):
         _check(LIB.ck_ctx_set_layout(self.h, layout))

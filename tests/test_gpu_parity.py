@@ -289,11 +289,13 @@ def test_descriptor_layout_rejects_oversized_shapes(ctx):
 def test_n2_20_one_and_two_buffer_passes_agree(ctx):
     """The rare-event HBM shape (n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
     seed 4).  One 16384-trial call is memory-capped, so it runs single-buffered
-    passes; 4096-trial calls run double-buffered.  Failures and probe totals
-    agree, and the first 24 trials match the oracle probe for probe."""
+    passes that insert the items in 32 windows; 4096-trial calls run
+    double-buffered without windows.  Failures and probe totals agree, and the
+    first 24 trials match the oracle probe for probe."""
     p, op = both("bgh25", 1 << 20, 20, k=3, buckets=1363149, kick=1000, stash=2)
     rk = ck.batch_key(4, 0)
     f1, st1 = ctx.launch(p, rk, 0, 16384)
+    assert st1.launches == 2 * 32 * -(-16384 // st1.chunk)
     f2, probes2 = [], 0
     for off in range(0, 16384, 4096):
         f, st = ctx.launch(p, rk, off, 4096)
