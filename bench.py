@@ -338,14 +338,14 @@ def graph_leg(ck, p, steps: int, trials: int):
 def hbm_leg(ck, trials: int, layout: int = 0, warm: int = 0):
     """Rare-event HBM shape: n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
     ell_i 20, seed 4 -- the first `trials` trials of batch 0, after an untimed
-    launch of `warm` trials of batch 1 (default: 2^15, more than one pass
+    launch of `warm` trials of batch 1 (default: 2^16, more than one pass
     holds, so that the workspace, 95 % of the free memory, is allocated outside
     the timed region).  trials = 0 times two full passes of the size that
     launch used."""
     p = ck.params("bgh25", 1 << 20, 20, k=3, buckets=1363149, kick=1000, stash=2)
     ctx = ck.Context(0)
     ctx.set_layout(layout)
-    _, ws = ctx.launch(p, ck.batch_key(4, 1), 0, warm or max(trials, 1 << 15))
+    _, ws = ctx.launch(p, ck.batch_key(4, 1), 0, warm or max(trials, 1 << 16))
     trials = trials or 2 * ws.chunk
     t0 = time.perf_counter()
     f, st = ctx.launch(p, ck.batch_key(4, 0), 0, trials)

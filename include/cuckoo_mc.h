@@ -99,7 +99,7 @@ int ck_ctx_set_tier(ck_ctx* ctx, uint32_t tier);
 /* Caps the trials per device pass (0 = library default). */
 int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
 /* Residual-layout item windows: each HBM-tier pass inserts the items in
- * windows of `items` (0 = automatic: windows of pnum/32 when memory caps a pass
+ * windows of `items` (0 = automatic: windows of pnum/64 when memory caps a pass
  * below its target, else none).  A window's build writes the candidates of its
  * items only, so a slot needs less memory and more trials fit in a pass.
  * Results do not depend on it.  Not applied in graph mode or to launches that

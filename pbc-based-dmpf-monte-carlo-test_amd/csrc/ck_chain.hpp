@@ -5,6 +5,8 @@
 //   u32: ordinal | choice << 30      u16: ordinal | choice << 14 (ordinal < 2^14)
 // and all-ones is empty.  map[o*K + w] is bucket(o, w), precomputed by the build.
 #pragma once
+#include <type_traits>
+
 #include "ck_device.hpp"
 
 namespace ck {
@@ -310,9 +312,63 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
 // the entry it just read and the slot it read it from.  (k-1)*bb + 2 bits fit
 // a u32 entry at n = 2^14, k = 3 (2*15 + 2 = 32, choice <= 2 keeps the entry
 // below all-ones) and a u64 entry up to n = 2^20 and beyond, where the full
-// descriptor (k*bb + 2 bits) no longer fits.  New items take their candidates
-// from a stream in insertion order: one packed u64 per item when k*bb <= 64,
-// else the item's k map entries; it is prefetched one item ahead.
+// descriptor (k*bb + 2 bits) no longer fits.  Between the two, 48-bit entries
+// (Entries48) take 6 bytes: n = 2^20, k = 3 needs 2*21 + 2 = 44 bits.  New
+// items take their candidates from a stream in insertion order: one packed u64
+// per item when k*bb <= 64, else the item's k map entries; it is prefetched
+// one item ahead.
+
+// A slot's table of 48-bit residual entries, 6 bytes apart from a 4-byte
+// aligned base; `t[b]` reads and writes entry b like an element of a u64
+// array.  A read loads the two aligned 32-bit words that cover the entry (they
+// reach at most 2 bytes past the last entry, so the table carries 8 bytes of
+// slack) and maps the 48-bit all-ones (empty) to the u64 all-ones.  A write is
+// a 32-bit and a 16-bit store, ordered by the entry's alignment (6b is 0 or 2
+// mod 4), so neighbouring entries are untouched.
+struct Entries48 {
+    char* base;
+    struct Ref {
+        char* a;
+        __device__ __forceinline__ operator uint64_t() const {
+            // pointer arithmetic, not an integer round trip, keeps the global address space
+            const uint32_t mis = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(a)) & 3u;
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(a - mis);
+            const uint64_t both = w[0] | (uint64_t(w[1]) << 32);
+            const uint64_t v = (both >> (mis * 8)) & 0xFFFFFFFFFFFFull;
+            return v == 0xFFFFFFFFFFFFull ? ~0ull : v;
+        }
+        __device__ __forceinline__ void operator=(uint64_t v) const {
+            if (reinterpret_cast<uintptr_t>(a) & 3) {
+                *reinterpret_cast<uint16_t*>(a) = static_cast<uint16_t>(v);
+                *reinterpret_cast<uint32_t*>(a + 2) = static_cast<uint32_t>(v >> 16);
+            } else {
+                *reinterpret_cast<uint32_t*>(a) = static_cast<uint32_t>(v);
+                *reinterpret_cast<uint16_t*>(a + 4) = static_cast<uint16_t>(v >> 32);
+            }
+        }
+    };
+    __device__ __forceinline__ Ref operator[](uint32_t b) const { return Ref{base + size_t(b) * 6}; }
+};
+
+// Residual table entry types of the kernels: u32, u64, or E48 (Entries48,
+// decoded as u64).
+struct E48 {};
+template <class E>
+struct ResWord {
+    using type = E;
+};
+template <>
+struct ResWord<E48> {
+    using type = uint64_t;
+};
+template <class E>
+__device__ __forceinline__ auto res_table(char* p) {
+    if constexpr (std::is_same<E, E48>::value)
+        return Entries48{p};
+    else
+        return reinterpret_cast<E*>(p);
+}
+
 template <int K, class E>
 struct ResCodec {
     uint32_t bb;     // bits per bucket index
@@ -368,9 +424,10 @@ struct ItemSpan {
 };
 
 // BGH25 over the residual layout (same decisions as chain_bgh).
-template <int K, class E, class M>
+// T is the table: an E* or Entries48 (E = u64).
+template <int K, class E, class M, class T>
 __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
-                                              E* __restrict__ t, ItemSpan span, uint64_t& walk, uint64_t& rnd,
+                                              T t, ItemSpan span, uint64_t& walk, uint64_t& rnd,
                                               uint32_t& stash_left, uint32_t& probes) {
     constexpr E kEmpty = static_cast<E>(~E(0));
     uint32_t item = span.begin, coll = 0, pr = 0;
@@ -409,10 +466,9 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
 }
 
 // DRRT18 over the residual layout (same decisions as chain_drrt).
-template <int K, class E, class M>
+template <int K, class E, class M, class T>
 __device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
-                                               E* __restrict__ t, ItemSpan span, uint32_t& stash_left,
-                                               uint32_t& probes) {
+                                               T t, ItemSpan span, uint32_t& stash_left, uint32_t& probes) {
     constexpr E kEmpty = static_cast<E>(~E(0));
     uint32_t pr = 0;
     bool failed = false;

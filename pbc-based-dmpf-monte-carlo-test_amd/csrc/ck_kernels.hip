@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_digest(Shape S, uint32_t count, const M
 }
 
 // chain (HBM tier, residual layout): M is the candidate stream (u64 packed
-// descriptor or u32 map rows), E the residual table entry (u32 or u64).  The
+// descriptor or u32 map rows), E the residual table entry (u32, u64 or E48).  The
 // launch inserts items [item0, item1), whose candidates the stream holds.
 // With `state` (a trial split into windows of items), the chain resumes from
 // and saves, per slot: the walk and random word, state = stash left | failed
@@ -207,9 +207,9 @@ __global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint
             before = probes_out[slot];
         }
         if (!(flags >> 31)) {
-            const ResCodec<K, E> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+            const ResCodec<K, typename ResWord<E>::type> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
             const CandStream<K, M> st{map + slot * map_stride, S.dbits, (1ull << S.dbits) - 1};
-            E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
+            const auto t = res_table<E>(tables + slot * table_bytes);
             const ItemSpan span{item0, item1};
             uint32_t stash_left = flags;
             if constexpr (BGH) {
@@ -236,12 +236,13 @@ __global__ __launch_bounds__(256) void k_digest_res(Shape S, uint32_t count, con
                                                     size_t table_bytes, unsigned long long* __restrict__ out) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= count) return;
-    const ResCodec<K, E> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
-    const E* t = reinterpret_cast<const E*>(tables + slot * table_bytes);
+    using W = typename ResWord<E>::type;
+    const ResCodec<K, W> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+    const auto t = res_table<E>(const_cast<char*>(tables) + slot * table_bytes);
     uint64_t sum = 0;
     for (uint32_t b = 0; b < S.buckets; ++b) {
-        const E e = t[b];
-        if (e == static_cast<E>(~E(0))) continue;
+        const W e = t[b];
+        if (e == static_cast<W>(~W(0))) continue;
         Cands c = {};
         const uint32_t choice = R.dec(e, b, c);
         uint64_t h = 0;
@@ -551,6 +552,7 @@ static bool res_fits(const ck_params& p, uint32_t width) {
 }
 static bool res_ok(const ck_params& p) { return res_fits(p, 64); }
 static bool res_e32(const ck_params& p) { return res_fits(p, 32); }
+static bool res_e48(const ck_params& p) { return res_fits(p, 48); }
 static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits <= 64; }
 
 // HBM-tier pass buffer: [counters | map C x map_bytes (x2 with windows) |
@@ -562,7 +564,7 @@ static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits 
 // chain of window j reads the other.
 struct HbmPlan {
     bool desc, map16, table16, xchg;
-    bool res, stream64, e32;  // residual layout: u64 candidate stream, u32 entries
+    bool res, stream64, e32, e48;  // residual layout: u64 candidate stream, u32 / 48-bit entries
     uint32_t window;     // items per map (pnum unless windowed)
     uint32_t nwin;       // windows per trial
     size_t map_stride;   // map entries per slot
@@ -581,14 +583,15 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uin
     h.res = layout == CK_LAYOUT_RESIDUAL;
     h.stream64 = h.res && res_stream64(p);
     h.e32 = h.res && res_e32(p);
+    h.e48 = h.res && !h.e32 && res_e48(p);
     h.map16 = !desc && !h.res && map16_ok(p);
     h.table16 = !desc && !h.res && !h.xchg && table16_ok(p);
     const bool m64 = desc || h.stream64;
     const size_t mw = m64 ? 8 : h.map16 ? 2 : 4;
-    const size_t tw = h.res ? (h.e32 ? 4 : 8) : desc ? 8 : h.table16 ? 2 : 4;
+    const size_t tw = h.res ? (h.e32 ? 4 : h.e48 ? 6 : 8) : desc ? 8 : h.table16 ? 2 : 4;
     h.map_bytes = align256(size_t(window) * (m64 ? 1 : p.k) * mw);
     h.map_stride = h.map_bytes / mw;
-    h.table_bytes = align256(size_t(p.buckets) * tw);
+    h.table_bytes = align256(size_t(p.buckets) * tw + (h.e48 ? 8 : 0));  // 48-bit reads reach 2 bytes past
     size_t o = 0;
     h.off_cnt = o;
     o = align256(o + CNT_WORDS * 4);
@@ -643,13 +646,16 @@ static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint
 // Item windows (residual layout).  When memory caps a single-buffered pass
 // below its target, each slot keeps the candidates of 2 x pnum / kWindows items
 // instead of pnum, and a pass runs as kWindows build + chain launches, each
-// chain stopping at an item boundary.  The candidate stream is 44 % of a slot
-// at n = 2^20 (8 of 19.3 MB); the memory that windows free holds more slots,
-// and these chains are latency-bound: a pass's chain time stays at 4.1 s from
-// 15,137 to 25,552 chains.  Measured at n = 2^20 (trials/s): no windows 3,425;
-// 8 windows 5,346; 16: 5,813; 32: 6,089.  Each window's build derives the
-// trial and its round cache again, under the previous window's chain.
-static constexpr uint32_t kWindows = 32;
+// chain stopping at an item boundary.  At n = 2^20 the candidate stream (one
+// u64 per item, 8.4 MB) is half of a slot; the memory that windows free holds
+// more slots, and these chains are latency-bound, so more chains in flight is
+// more throughput.  Measured at n = 2^20 with u64 entries (trials/s): no
+// windows 3,425 (15,137 chains per pass); 8 windows 5,346; 16: 5,813; 32: 6,089
+// (25,552 chains, the same 4.1 s of chain per pass).  With 48-bit entries:
+// 32 windows 6,960 (33,556 chains, 4.75 s of chain per pass); 64: 7,146.  Each
+// window's build derives the trial and its round cache again, under the
+// previous window's chain.
+static constexpr uint32_t kWindows = 64;
 static uint32_t stream_window(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout, bool allowed) {
     if (!allowed || layout != CK_LAYOUT_RESIDUAL) return p.pnum;
     if (c.window) return std::min(c.window, p.pnum);
@@ -771,6 +777,9 @@ static void launch_digest_k(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (h.res && h.e32)
         st(&k_digest_res<K, uint32_t>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
            a.table_bytes, a.digests);
+    else if (h.res && h.e48)
+        st(&k_digest_res<K, E48>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
+           a.table_bytes, a.digests);
     else if (h.res)
         st(&k_digest_res<K, uint64_t>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
            a.table_bytes, a.digests);
@@ -797,10 +806,14 @@ template <int K, bool BGH>
 static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (h.res && h.stream64 && h.e32)
         launch_chain_res_t<K, BGH, uint64_t, uint32_t>(a, st);
+    else if (h.res && h.stream64 && h.e48)
+        launch_chain_res_t<K, BGH, uint64_t, E48>(a, st);
     else if (h.res && h.stream64)
         launch_chain_res_t<K, BGH, uint64_t, uint64_t>(a, st);
     else if (h.res && h.e32)
         launch_chain_res_t<K, BGH, uint32_t, uint32_t>(a, st);
+    else if (h.res && h.e48)
+        launch_chain_res_t<K, BGH, uint32_t, E48>(a, st);
     else if (h.res)
         launch_chain_res_t<K, BGH, uint32_t, uint64_t>(a, st);
     else if (h.desc && h.xchg)

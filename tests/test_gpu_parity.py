@@ -288,17 +288,19 @@ def test_descriptor_layout_rejects_oversized_shapes(ctx):
 
 def test_n2_20_one_and_two_buffer_passes_agree(ctx):
     """The rare-event HBM shape (n = 2^20, m = 1.3n, k = 3, stash 2, kick 1000,
-    seed 4).  One 16384-trial call is memory-capped, so it runs single-buffered
-    passes that insert the items in 32 windows; 4096-trial calls run
-    double-buffered without windows.  Failures and probe totals agree, and the
-    first 24 trials match the oracle probe for probe."""
+    seed 4).  One 32768-trial call is memory-capped (a slot holding every
+    item's candidates is 16.6 MB), so it runs single-buffered passes that
+    insert the items in 64 windows; 8192-trial calls run double-buffered
+    without windows.  Failures and probe totals agree, and the first 24 trials
+    match the oracle probe for probe."""
     p, op = both("bgh25", 1 << 20, 20, k=3, buckets=1363149, kick=1000, stash=2)
     rk = ck.batch_key(4, 0)
-    f1, st1 = ctx.launch(p, rk, 0, 16384)
-    assert st1.launches == 2 * 32 * -(-16384 // st1.chunk)
+    f1, st1 = ctx.launch(p, rk, 0, 32768)
+    assert st1.launches == 2 * 64 * -(-32768 // st1.chunk)
     f2, probes2 = [], 0
-    for off in range(0, 16384, 4096):
-        f, st = ctx.launch(p, rk, off, 4096)
+    for off in range(0, 32768, 8192):
+        f, st = ctx.launch(p, rk, off, 8192)
+        assert st.launches == 2 * -(-8192 // st.chunk)
         f2 += list(map(int, f))
         probes2 += st.probes
     assert list(map(int, f1)) == f2
@@ -310,10 +312,12 @@ def test_n2_20_one_and_two_buffer_passes_agree(ctx):
 
 RESIDUAL_SHAPES = [
     ("bgh25", 16384, 16, {"kick": 500}, 64),                                   # u32 entries, 2*15 + 2 = 32 bits
-    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50, "stash": 1}, 256),  # 3*10 + 2 = 32 bits, k = 4: u64
-    ("bgh25", 60000, 16, {"k": 4, "buckets": 70000, "kick": 200, "stash": 2}, 24),  # 4*17 > 64: u32 map stream
-    ("drrt18", 60000, 16, {"k": 4, "buckets": 70000, "kick": 100}, 24),
-    ("bgh25", 1 << 20, 20, {"buckets": 1363149, "kick": 1000, "stash": 2}, 6),   # n = 2^20: u64 entries
+    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50, "stash": 1}, 256),  # 3*10 + 2 = 32 bits, k = 4: 48-bit
+    ("bgh25", 600, 12, {"k": 4, "buckets": 1000, "kick": 50}, 256),             # 48-bit entries
+    ("bgh25", 30000, 16, {"buckets": 40000, "kick": 300, "stash": 1}, 24),      # 2*16 + 2 = 34 bits: 48-bit
+    ("bgh25", 60000, 16, {"k": 4, "buckets": 70000, "kick": 200, "stash": 2}, 24),  # 4*17 > 64: u32 map stream, u64
+    ("drrt18", 60000, 16, {"k": 4, "buckets": 70000, "kick": 100}, 24),            # 3*17 + 2 = 53 bits: u64 entries
+    ("bgh25", 1 << 20, 20, {"buckets": 1363149, "kick": 1000, "stash": 2}, 6),   # n = 2^20: 48-bit entries
 ]
 
 
