@@ -4,6 +4,8 @@ Bit-exact: failure offsets, per-trial probe counts, final tables, the device
 key derivation and bucket map.  Sizes are chosen so the oracle finishes in
 seconds."""
 import hashlib
+import json
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -162,6 +164,22 @@ def test_headline_shape_prefix(ctx, tier):
     of, oprobes, _ = ock.run_range(ock.batch_key(1, 0), 0, count, op, probes=True)
     assert list(f) == list(of)
     assert (probes == oprobes).all()
+
+
+def test_headline_shape_whole_batch0(ctx):
+    """All 2^20 trials of batch 0 at the headline shape (seed 1).  With the
+    reference's own kick bound (3n = 49152) the failing offsets equal those
+    the reference's cpu_bench wrote; with kick 500 the failing offsets and the
+    probe total equal the oracle's."""
+    h = json.loads((Path(__file__).parent / "golden" / "headline_batch0.json").read_text())
+    count = h["shape"]["count"]
+    p_ref = ck.params("bgh25", 16384, 16)
+    assert p_ref.kick == h["reference"]["kick"]
+    f, _ = ctx.launch(p_ref, ck.batch_key(1, 0), 0, count)
+    assert list(map(int, f)) == h["reference"]["failures"]
+    g, st = ctx.launch(ck.params("bgh25", 16384, 16, kick=500), ck.batch_key(1, 0), 0, count)
+    assert list(map(int, g)) == h["kick500"]["failures"]
+    assert st.probes == h["kick500"]["probes"]
 
 
 def test_chunking_does_not_change_results(ctx, tier):
