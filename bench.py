@@ -75,6 +75,32 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def host_limits() -> dict:
+    """What bounds the CPU legs on this host: logical CPUs, the affinity mask
+    and the cgroup CPU quota (cgroup v2 cpu.max or v1 cfs quota / period)."""
+    out = {"nproc": os.cpu_count()}
+    try:
+        out["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    for path in ("/sys/fs/cgroup/cpu.max", "/sys/fs/cgroup/cpu/cpu.cfs_quota_us"):
+        try:
+            text = Path(path).read_text().strip()
+        except OSError:
+            continue
+        if path.endswith("cpu.max"):
+            quota, _, period = text.partition(" ")
+        else:
+            quota = text
+            try:
+                period = Path("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read_text().strip()
+            except OSError:
+                period = ""
+        out["cgroup_cpu_quota"] = None if quota in ("max", "-1") or not period else int(quota) / int(period)
+        break
+    return out
+
+
 def cpu_baseline(ctx, ck, p, seconds: float):
     """Oracle rate on a bounded prefix of the workload; also the parity check."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -102,7 +128,7 @@ def cpu_baseline(ctx, ck, p, seconds: float):
         "sample": f"first {sample} trials of batch 0 (seed 1), oracle/ck_oracle.c with AES-NI round cache, "
                   f"{cores} threads on {cpu_model()}",
         "seconds": dt, "failures": len(of), "gpu_failures_equal": parity,
-        "single_core_trials_per_s": one_rate,
+        "single_core_trials_per_s": one_rate, "host": host_limits(),
         "reference_cpu_bench": cpu_reference(cores, len(g15)),
     }, parity
 
