@@ -319,26 +319,42 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
 // one item ahead.
 
 // A slot's table of 48-bit residual entries, 6 bytes apart from a 4-byte
-// aligned base; `t[b]` reads and writes entry b like an element of a u64
-// array.  A read loads the two aligned 32-bit words that cover the entry (they
-// reach at most 2 bytes past the last entry, so the table carries 8 bytes of
-// slack) and maps the 48-bit all-ones (empty) to the u64 all-ones.  A write is
-// a 32-bit and a 16-bit store, ordered by the entry's alignment (6b is 0 or 2
-// mod 4), so neighbouring entries are untouched.
+// aligned base.  `t[b]` is a handle to entry b that reads and writes like an
+// element of a u64 array.  A read loads the 8 bytes at the dword-aligned
+// address at or 2 bytes below the entry (so the table carries 8 bytes of slack
+// past the last entry) and maps the 48-bit all-ones (empty) to the u64
+// all-ones.  A write through a handle that has been read stores the same 8
+// bytes back with the entry replaced: the 2 neighbouring bytes are rewritten
+// with the values just read, which is safe because one lane owns a table and
+// writes it only through the handle of the probe in progress.  A write before
+// any read is a 16-bit and a 32-bit store that leave the neighbours untouched.
 struct Entries48 {
     char* base;
     struct Ref {
         char* a;
+        mutable uint64_t raw = 0;  // the 8 bytes read, once read
+        mutable bool read = false;
+        __device__ __forceinline__ uint32_t mis() const {
+            return static_cast<uint32_t>(reinterpret_cast<uintptr_t>(a)) & 3u;
+        }
         __device__ __forceinline__ operator uint64_t() const {
             // pointer arithmetic, not an integer round trip, keeps the global address space
-            const uint32_t mis = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(a)) & 3u;
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(a - mis);
-            const uint64_t both = w[0] | (uint64_t(w[1]) << 32);
-            const uint64_t v = (both >> (mis * 8)) & 0xFFFFFFFFFFFFull;
+            const uint32_t m = mis();
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(a - m);
+            raw = w[0] | (uint64_t(w[1]) << 32);
+            read = true;
+            const uint64_t v = (raw >> (m * 8)) & 0xFFFFFFFFFFFFull;
             return v == 0xFFFFFFFFFFFFull ? ~0ull : v;
         }
         __device__ __forceinline__ void operator=(uint64_t v) const {
-            if (reinterpret_cast<uintptr_t>(a) & 3) {
+            const uint32_t m = mis();
+            if (read) {
+                const uint64_t keep = ~(0xFFFFFFFFFFFFull << (m * 8));
+                const uint64_t x = (raw & keep) | ((v & 0xFFFFFFFFFFFFull) << (m * 8));
+                uint32_t* w = reinterpret_cast<uint32_t*>(a - m);
+                w[0] = static_cast<uint32_t>(x);
+                w[1] = static_cast<uint32_t>(x >> 32);
+            } else if (m) {
                 *reinterpret_cast<uint16_t*>(a) = static_cast<uint16_t>(v);
                 *reinterpret_cast<uint32_t*>(a + 2) = static_cast<uint32_t>(v >> 16);
             } else {
@@ -439,7 +455,8 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
         if (rnd < K) rnd = splitmix(walk);
         const uint32_t which = take_choice<K>(rnd);
         const uint32_t b = pick<K>(c, which);
-        const E old = t[b];
+        auto&& slot = t[b];
+        const E old = slot;
         ++pr;
         bool advance = old == kEmpty;
         if (!advance && ++coll > S.kick) {
@@ -450,7 +467,7 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
             --stash_left;  // the item in hand goes to the stash; nothing is written
             advance = true;
         } else {
-            t[b] = R.enc(c, which);
+            slot = R.enc(c, which);
         }
         if (advance) {
             ++item;
@@ -482,18 +499,20 @@ __device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K,
         for (int w = 0; w < K; ++w) {
             if (placed) break;
             ++pr;
-            if (t[c[w]] == kEmpty) {
-                t[c[w]] = R.enc(c, uint32_t(w));
+            auto&& slot = t[c[w]];
+            if (static_cast<E>(slot) == kEmpty) {
+                slot = R.enc(c, uint32_t(w));
                 placed = true;
             }
         }
         if (placed) continue;
         uint32_t choice = 0, target = c[0], kicks = 0;
         for (;;) {
-            const E disp = t[target];
+            auto&& slot = t[target];
+            const E disp = slot;
             ++pr;
             if (disp == kEmpty) {
-                t[target] = R.enc(c, choice);
+                slot = R.enc(c, choice);
                 break;
             }
             if (kicks >= S.kick) {
@@ -504,7 +523,7 @@ __device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K,
                 failed = true;
                 break;
             }
-            t[target] = R.enc(c, choice);
+            slot = R.enc(c, choice);
             choice = R.dec(disp, target, c);
             const uint32_t prev = target;
             bool found = false;
