@@ -59,8 +59,9 @@ enum {
     CK_LAYOUT_DESC = 2,       /* entry = occupant's packed candidates | choice (k*bits+2 <= 63); exchange probes */
     CK_LAYOUT_MAP_XCHG = 3,   /* u32 entry = ordinal | choice; exchange probes */
     CK_LAYOUT_DESC_PLAIN = 4, /* descriptor entries; load + store probes */
-    CK_LAYOUT_RESIDUAL = 5    /* entry = occupant's other k-1 candidates | choice (u32 when it fits, else u64),
-                                 its current bucket implied by the slot; load + store probes */
+    CK_LAYOUT_RESIDUAL = 5    /* entry = occupant's other k-1 candidates | choice (u32 when it fits, else
+                                 48 bits in 6 bytes when it fits, else u64), its current bucket implied by
+                                 the slot; load + store probes */
 };
 
 typedef struct ck_stats {
