@@ -37,11 +37,17 @@ SPLITMIX_OPS = 22      # one SplitMix64 draw (int32-equivalent)
 CACHED_ROUND_OPS = 4   # one cached Feistel round (lookup index, xor, mask, swap)
 EVAL_FIXED_OPS = 6     # element, y = k*e+w, cycle-walk test, division
 PROBE_OPS = 20         # choice draw (div/mod by k), table read, compare, write, advance
-# Algorithmic bytes of k_chain in the descriptor layout (8-byte slots): a probe
-# reads its slot and writes it back (placement or eviction), and each new item's
-# 8-byte descriptor is read once from the insertion-order stream.
-PROBE_BYTES = 16
-ITEM_BYTES = 8
+# Algorithmic bytes of the insertion chain: a probe reads one u32 slot of the
+# reference's table (ordinal | choice << 16) and writes at most one back, 8 B;
+# a trial's table starts as buckets u32 empties, 4 B per bucket (written by the
+# build, or skipped through pass tags, so not part of k_chain's launch).
+ALGO_PROBE_BYTES = 8
+ALGO_BUCKET_INIT_BYTES = 4
+# The bytes this build's own descriptor layout moves per probe and per item:
+# an 8-byte slot read and written back, and each new item's 8-byte descriptor
+# read once from the insertion-order stream (reported beside the roofline).
+OWN_PROBE_BYTES = 16
+OWN_ITEM_BYTES = 8
 # HBM bytes per probe of k_chain in the descriptor layout with load + store
 # probes (the default at this shape), from FETCH_SIZE and WRITE_SIZE per
 # dispatch divided by the dispatch's probes; the chain's reads are 64-byte
@@ -51,18 +57,24 @@ TRAFFIC_WRITE_PER_PROBE = 27.48
 
 
 def int_ops_per_trial(p, probes_per_trial: float) -> float:
-    accept = (3 << p.ell_i) / p.domain
+    """Modelled integer ops per trial: two AES blocks, a key schedule and two
+    SplitMix64 draws of the derivation, the 8 * 2^half AES blocks of the round
+    cache, and per bucket evaluation (E = P for bgh25) 8 cached Feistel rounds
+    per cycle-walk pass plus the fixed part, plus the probe itself.  A pass
+    lands in the domain with probability domain / 2^(2 * half_bits)."""
+    accept = p.domain / float(1 << (2 * p.half_bits))
     cache = (8 << p.half_bits) * AES_OPS
     chain = probes_per_trial * (8 / accept * CACHED_ROUND_OPS + EVAL_FIXED_OPS) + probes_per_trial * PROBE_OPS
     return 2 * AES_OPS + KEYSCHED_OPS + 2 * SPLITMIX_OPS + cache + chain
 
 
 def host_cores() -> int:
+    """Every host core this process may run on (its affinity mask)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, n)
 
 
 def cpu_model() -> str:
@@ -159,23 +171,32 @@ def cpu_reference(cores: int, gpu_failures_prefix):
 
 
 def random_access_ceiling(probes_per_s: float):
-    """Runs lib/ck_randbench: the chain's access pattern (65536 lanes, each a
-    dependent random read + write of one 8-byte slot of its own 192 KB table)
-    without the chain's logic.  Its rate is the achievable ceiling for the
-    insertion chains on this GPU."""
+    """Runs lib/ck_randbench: the chain's access pattern (each lane a dependent
+    random read + write of one 8-byte slot of its own 192 KB table, with one
+    XOR and one 32-bit multiply-high between a load and the next address)
+    without the chain's logic, at 32K..256K lanes.  The best read + write rate
+    is the achievable ceiling for the insertion chains on this GPU."""
     import re
     import subprocess
     from paths import PKG
     exe = PKG / "lib" / "ck_randbench"
     if not exe.exists():
         return None
-    out = subprocess.run([str(exe), "65536", "196608", "32768"], capture_output=True, text=True, timeout=300)
-    rates = dict(re.findall(r"^(read_write|read_only) .*steps_per_s=([0-9.e+]+)", out.stdout, re.M))
-    if out.returncode != 0 or "read_write" not in rates:
+    out = subprocess.run([str(exe), "196608", "16384", "32768", "65536", "131072", "262144"],
+                         capture_output=True, text=True, timeout=300)
+    rows = re.findall(r"^(read_write|read_only) lanes=(\d+) .*steps_per_s=([0-9.e+]+)", out.stdout, re.M)
+    if out.returncode != 0 or not rows:
         return {"error": out.stderr[-300:]}
-    rw, ro = float(rates["read_write"]), float(rates.get("read_only", 0))
-    return {"pattern": "65536 lanes x 192 KB tables, dependent random 8-byte read + write per step",
-            "read_write_steps_per_s": rw, "read_only_steps_per_s": ro,
+    sweep = {}
+    for form, lanes, rate in rows:
+        sweep.setdefault(int(lanes), {})[form] = float(rate)
+    best_lanes = max(sweep, key=lambda n: sweep[n].get("read_write", 0))
+    rw = sweep[best_lanes]["read_write"]
+    return {"pattern": "lanes x 192 KB tables, dependent random 8-byte read + write per step "
+                       "(load -> xor -> mul_hi -> address)",
+            "read_write_steps_per_s": rw, "best_lanes": best_lanes,
+            "read_only_steps_per_s": max(v.get("read_only", 0) for v in sweep.values()),
+            "sweep": [{"lanes": n, **sweep[n]} for n in sorted(sweep)],
             "chain_probes_per_s": probes_per_s, "frac": probes_per_s / rw}
 
 
@@ -389,8 +410,11 @@ def hbm_leg(ck, trials: int, layout: int = 0, warm: int = 0):
         # Chains are latency-bound, so a pass takes about as long whatever its trial
         # count: a long run of full passes goes at trials_per_pass / pass time.
         "full_pass_trials_per_s": st.chunk * passes / dt,
-        "chain_algorithmic_gbs": ((PROBE_BYTES * st.probes + ITEM_BYTES * p.pnum * trials) / chain_s / 1e9
-                                  if chain_s > 0 else None),
+        # Algorithmic bytes of the shape (8 per probe, 4 per bucket of each
+        # trial's initial table) over the wall time, and the chain's share
+        # (8 per probe) over the chains' device time.
+        "algorithmic_gbs": (ALGO_PROBE_BYTES * st.probes + ALGO_BUCKET_INIT_BYTES * p.buckets * trials) / dt / 1e9,
+        "chain_algorithmic_gbs": ALGO_PROBE_BYTES * st.probes / chain_s / 1e9 if chain_s > 0 else None,
         "device_ms": {"build": st.build_ms, "chain": st.chain_ms},
     }
 
@@ -493,10 +517,12 @@ def main():
     value = trials / elapsed
     chain_launches = max(1, agg["launches"] // 2)  # build + chain per pass
     chain_avg_s = agg["chain_ms"] / 1e3 / chain_launches
-    bytes_per_launch = (PROBE_BYTES * agg["probes"] + ITEM_BYTES * p.pnum * trials) / chain_launches
+    probes_per_launch = agg["probes"] / chain_launches
+    bytes_per_launch = ALGO_PROBE_BYTES * probes_per_launch
     achieved = bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else 0.0
+    own_bytes_per_launch = (OWN_PROBE_BYTES * agg["probes"] + OWN_ITEM_BYTES * p.pnum * trials) / chain_launches
     measured_layout = args.layout in ("auto", "desc-plain")
-    traffic = ((TRAFFIC_READ_PER_PROBE + TRAFFIC_WRITE_PER_PROBE) * agg["probes"] / chain_launches
+    traffic = ((TRAFFIC_READ_PER_PROBE + TRAFFIC_WRITE_PER_PROBE) * probes_per_launch
                if measured_layout else None)
     probes_per_trial = agg["probes"] / trials
     ops = int_ops_per_trial(p, probes_per_trial)
@@ -525,11 +551,18 @@ def main():
             "kernel": "k_chain<3,bgh> (insertion chains, one lane per trial)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "avg_launch_ms": chain_avg_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": chain_avg_s * 1e3, "probes_per_launch": probes_per_launch,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "achieved_note": "8 algorithmic bytes per probe: one u32 slot of the reference's table read and at "
+                             "most one written",
+            "own_slot_bytes_per_launch": own_bytes_per_launch,
+            "own_slot_gbs": own_bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else None,
+            "own_slot_note": "this build's descriptor layout: 16 B per probe (an 8-byte slot read and written "
+                             "back) and 8 B per inserted item (its descriptor)",
+            "traffic_source": "profile-derived, not measured in this run: (FETCH_SIZE + WRITE_SIZE) per probe "
+                              "of k_chain from rocprofv3 --pmc (profiles/k_chain_hbm_n16384_desc_traffic.md) "
+                              "times this run's probes per launch",
             "traffic_rate_gbs": traffic / chain_avg_s / 1e9 if traffic and chain_avg_s > 0 else None,
-            "note": "achieved counts 16 algorithmic bytes per probe (8-byte slot read and write) and 8 per "
-                    "inserted item (its descriptor); each probe costs one random 64-byte DRAM read and ~0.86 "
-                    "32-byte write-backs, and the chain is bound by that random-access rate",
         },
         "int_ops_model": {
             "ops_per_trial": ops, "achieved_tops": ops * value / 1e12, "peak_tops": VALU_PEAK_TOPS,
@@ -537,7 +570,7 @@ def main():
         },
         "device_ms": {"build": agg["build_ms"], "chain": agg["chain_ms"]},
     }
-    rec["roofline"]["random_access_ceiling"] = random_access_ceiling(agg["probes"] / chain_launches / chain_avg_s)
+    rec["roofline"]["random_access_ceiling"] = random_access_ceiling(probes_per_launch / chain_avg_s)
     ok = True
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)

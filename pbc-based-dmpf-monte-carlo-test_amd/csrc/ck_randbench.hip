@@ -1,42 +1,65 @@
-// ck_randbench.hip -- the achievable rate of the insertion chain's memory
-// access pattern on this GPU, without the chain's logic: every lane walks its
-// own table (by default 65536 lanes x 192 KB, the tables of one headline
-// pass), and each step reads one random 8-byte slot of it and writes the slot
-// back changed, the next slot depending on the value read (a dependent chain,
-// as in cuckoo insertion).  Prints steps/s for the read+write form and for a
-// read-only form; bench.py divides the chain's probes/s by the first.
+// ck_randbench.hip -- the rate the memory system gives the insertion chain's
+// access pattern, with nothing else on the dependent path.
 //
-//   ck_randbench [lanes] [table_bytes] [steps]
+// Every lane owns a table of 8-byte slots (by default 192 KB, the descriptor
+// table of one n = 2^14 trial) filled with random words.  A step reads one
+// slot and, in the read+write form, writes it back changed; the next slot is
+// mulhi32(low word of the value read ^ ctr, slots) with ctr advanced by a
+// constant every step.  So the dependent path is load -> one XOR -> one
+// 32-bit multiply-high -> address -> load, which is no longer than the chain's
+// (load -> shift/mask of a bucket field -> address -> load), and ctr keeps the
+// walk from settling into a short cycle of cached slots (a fixed random map of
+// 24,576 slots would cycle within ~200 steps).  Tables are 256-byte aligned
+// and disjoint, as the chain's are.
+//
+//   ck_randbench [table_bytes] [steps] [lanes ...]
+//
+// For each lane count it prints one line per form:
+//   read_write lanes=L table_bytes=B steps=S ms=T steps_per_s=R
+//   read_only  ...
+// The best read_write rate over the lane counts is the ceiling that bench.py
+// sets the chain's probes/s against.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 namespace {
 
-__device__ __forceinline__ uint64_t mix(uint64_t z) {
+constexpr uint32_t kGolden32 = 0x9E3779B9u;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
 
+// Fills words [0, n) with mix64(index + salt): random table contents.
+__global__ __launch_bounds__(256) void k_fill(uint64_t* __restrict__ buf, uint64_t n, uint64_t salt) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        buf[i] = mix64(i + salt);
+}
+
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64_t slots_per_lane, uint32_t lanes,
-                                              uint32_t steps, unsigned long long* __restrict__ sink) {
+__global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64_t pitch, uint32_t slots,
+                                              uint32_t lanes, uint32_t steps, unsigned long long* __restrict__ sink) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= lanes) return;
-    uint64_t* t = buf + uint64_t(lane) * slots_per_lane;
-    uint64_t state = mix(lane + 1);
+    uint64_t* t = buf + uint64_t(lane) * pitch;
+    uint32_t ctr = lane * kGolden32;
+    uint32_t i = __umulhi(static_cast<uint32_t>(mix64(lane)), slots);
     uint64_t acc = 0;
     for (uint32_t s = 0; s < steps; ++s) {
-        const uint64_t i = state % slots_per_lane;
         const uint64_t v = t[i];
-        if constexpr (WRITE) t[i] = v + state;
+        if constexpr (WRITE) t[i] = v ^ ctr;
         acc += v;
-        state = mix(state ^ v);
+        i = __umulhi(static_cast<uint32_t>(v) ^ ctr, slots);
+        ctr += kGolden32;
     }
-    if (acc == 0x5eed) sink[0] = acc;  // keeps the loads live
+    if (acc == 0x5eedull) sink[0] = acc;  // keeps the loads live
 }
 
 #define CHECK(x)                                                                              \
@@ -51,39 +74,58 @@ __global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64
 }  // namespace
 
 int main(int argc, char** argv) {
-    const uint32_t lanes = argc > 1 ? uint32_t(std::strtoul(argv[1], nullptr, 10)) : 65536u;
-    const uint64_t table = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 196608ull;
-    const uint32_t steps = argc > 3 ? uint32_t(std::strtoul(argv[3], nullptr, 10)) : 32768u;
-    if (lanes == 0 || table < 8 || steps == 0 || (table & 7)) {
-        std::fprintf(stderr, "usage: ck_randbench [lanes] [table_bytes, multiple of 8] [steps]\n");
+    const uint64_t table = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 196608ull;
+    const uint32_t steps = argc > 2 ? uint32_t(std::strtoul(argv[2], nullptr, 10)) : 16384u;
+    std::vector<uint32_t> lane_counts;
+    for (int a = 3; a < argc; ++a) lane_counts.push_back(uint32_t(std::strtoul(argv[a], nullptr, 10)));
+    if (lane_counts.empty()) lane_counts = {32768u, 65536u, 131072u, 262144u};
+    uint32_t max_lanes = 0;
+    for (uint32_t l : lane_counts) max_lanes = l > max_lanes ? l : max_lanes;
+    if (table < 8 || (table & 7) || table / 8 > 0xffffffffull || steps == 0 || max_lanes == 0 ||
+        max_lanes > (1u << 22)) {
+        std::fprintf(stderr, "usage: ck_randbench [table_bytes, multiple of 8] [steps] [lanes <= 2^22 ...]\n");
         return 2;
     }
-    const uint64_t slots = table / 8;
+    const uint32_t slots = static_cast<uint32_t>(table / 8);
+    const uint64_t pitch = ((table + 255) & ~uint64_t(255)) / 8;  // words per lane table
+    const uint64_t words = pitch * max_lanes;
+    size_t free_b = 0, total_b = 0;
+    CHECK(hipMemGetInfo(&free_b, &total_b));
+    if (words * 8 > free_b / 10 * 9) {
+        std::fprintf(stderr, "tables need %.1f GiB, more than 90%% of the free %.1f GiB\n", words * 8 / 1073741824.0,
+                     free_b / 1073741824.0);
+        return 2;
+    }
     uint64_t* buf = nullptr;
     unsigned long long* sink = nullptr;
-    CHECK(hipMalloc(&buf, uint64_t(lanes) * table));
+    CHECK(hipMalloc(&buf, words * 8));
     CHECK(hipMalloc(&sink, sizeof(unsigned long long)));
-    CHECK(hipMemset(buf, 0, uint64_t(lanes) * table));
+    hipLaunchKernelGGL(k_fill, dim3(8192), dim3(256), 0, 0, buf, words, 0x243F6A8885A308D3ull);
+    CHECK(hipGetLastError());
+    CHECK(hipDeviceSynchronize());
     hipEvent_t a, b;
     CHECK(hipEventCreate(&a));
     CHECK(hipEventCreate(&b));
-    const dim3 grid((lanes + 255) / 256), block(256);
-    for (int form = 0; form < 2; ++form) {
-        for (int rep = 0; rep < 2; ++rep) {  // the first run warms the TLB and clocks
-            CHECK(hipEventRecord(a));
-            if (form == 0)
-                hipLaunchKernelGGL(k_walk<true>, grid, block, 0, 0, buf, slots, lanes, steps, sink);
-            else
-                hipLaunchKernelGGL(k_walk<false>, grid, block, 0, 0, buf, slots, lanes, steps, sink);
-            CHECK(hipGetLastError());
-            CHECK(hipEventRecord(b));
-            CHECK(hipEventSynchronize(b));
+    for (uint32_t lanes : lane_counts) {
+        const dim3 grid((lanes + 255) / 256), block(256);
+        for (int form = 0; form < 2; ++form) {
+            float ms = 0;
+            for (int rep = 0; rep < 2; ++rep) {  // the first run warms the TLB and clocks
+                CHECK(hipEventRecord(a));
+                if (form == 0)
+                    hipLaunchKernelGGL(k_walk<true>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+                else
+                    hipLaunchKernelGGL(k_walk<false>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+                CHECK(hipGetLastError());
+                CHECK(hipEventRecord(b));
+                CHECK(hipEventSynchronize(b));
+                CHECK(hipEventElapsedTime(&ms, a, b));
+            }
+            const double rate = double(lanes) * steps / (ms / 1e3);
+            std::printf("%s lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n",
+                        form == 0 ? "read_write" : "read_only", lanes, (unsigned long long)table, steps, ms, rate);
+            std::fflush(stdout);
         }
-        float ms = 0;
-        CHECK(hipEventElapsedTime(&ms, a, b));
-        const double rate = double(lanes) * steps / (ms / 1e3);
-        std::printf("%s lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n",
-                    form == 0 ? "read_write" : "read_only", lanes, (unsigned long long)table, steps, ms, rate);
     }
     CHECK(hipFree(buf));
     CHECK(hipFree(sink));

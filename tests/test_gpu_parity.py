@@ -295,7 +295,7 @@ def test_descriptor_trial_map(ctx):
 
 
 def test_descriptor_layout_rejects_oversized_shapes(ctx):
-    p = ck.params("bgh25", 1 << 20, 20, buckets=1363148, kick=1000, stash=2)  # 3 x 21 bits + 2 > 63
+    p = ck.params("bgh25", 1 << 20, 20, buckets=1363149, kick=1000, stash=2)  # 3 x 21 bits + 2 > 63
     ctx.set_layout(ck.LAYOUT_DESC)
     try:
         with pytest.raises(ck.CkError):

@@ -96,8 +96,9 @@ def test_config_shapes():
     # n = 2^14, m = 24576, ell_i 16: half 9, bucket_size 8, domain 196608.
     p = ock.parameters("bgh25", 16384, 16, kick=500)
     assert (p.buckets, p.kick, p.bucket_size, p.domain, p.half_bits) == (24576, 500, 8, 196608, 9)
-    p = ock.parameters("bgh25", 1 << 20, 20, buckets=1363148, kick=1000, stash=2)
-    assert (p.bucket_size, p.domain, p.half_bits) == (3, 4089444, 11)
+    # n = 2^20 with m = ceil(1.3 n) = 1,363,149 (1.3 n = 1,363,148.8).
+    p = ock.parameters("bgh25", 1 << 20, 20, buckets=1363149, kick=1000, stash=2)
+    assert (p.bucket_size, p.domain, p.half_bits) == (3, 4089447, 11)
 
 
 def test_parameters_match_reference(fixtures):
