@@ -90,6 +90,26 @@ void ck_batch_key(uint64_t seed, uint64_t batch_id, uint32_t round_keys[44]);
 /* Host AES-128 on four big-endian words (host_aes128, cuckoo_bench.cu:360-392). */
 void ck_host_aes(const uint32_t round_keys[44], const uint32_t in[4], uint32_t out[4]);
 
+/* Inputs of one trial for the device-level trial API (include/cuckoo_trial.hpp),
+ * derived on the host exactly as the library's build kernels derive them
+ * (cuckoo_bench.cu:222-238): PRP key = schedule of AES_batch("KEY1", 0,
+ * offset, "PRP1"); (a, b, c, d) = AES_batch("WALK", 0, offset, "RNG1"); the
+ * walk starts at a<<32|b and randomness is its first SplitMix64 draw (the walk
+ * advanced once); shift = (c<<32|d) mod 2^ell_i and step = (SplitMix64 of
+ * shift ^ 0xd1b54a32d192ed03, | 1) mod 2^ell_i for affine input. */
+typedef struct ck_trial_inputs {
+    uint32_t prp_key[44];
+    uint64_t walk, randomness, shift, step;
+} ck_trial_inputs;
+int ck_trial_inputs_of(const ck_params* p, const uint32_t batch_round_keys[44], uint32_t offset,
+                       ck_trial_inputs* out);
+
+/* The trial's round cache (gpu_shared.cuh:68-77, cpu_trials.hpp:139-142):
+ * rounds[r << half_bits | right] = word 3 of AES_prp("CESP", r, half_bits,
+ * right), masked to half_bits bits, for r < 8 and right < 2^half_bits.
+ * rounds holds 8 << half_bits entries; CK_EINVAL when half_bits > 16. */
+int ck_round_cache(const ck_params* p, const uint32_t prp_key[44], uint16_t* rounds);
+
 /* One context per host thread and device (gpu_worker, cuckoo_bench.cu:813-973).
  * Uploads the AES tables to the device (upload_aes_tables, :64-75). */
 int ck_ctx_create(int device, ck_ctx** out);
@@ -123,6 +143,50 @@ int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
  * returns success but the memory stays in use until the process ends): use
  * graph mode in long-lived contexts. */
 int ck_ctx_set_graph(ck_ctx* ctx, uint32_t on);
+
+/* How bucket evaluations get their Feistel round function. */
+enum {
+    CK_PRP_AUTO = 0,   /* a round cache in LDS when half_bits <= 12, else AES evaluated directly */
+    CK_PRP_CACHE = 1,  /* a round cache (CK_EINVAL at launch when half_bits > 12) */
+    CK_PRP_DIRECT = 2  /* AES evaluated for every round (the reference's --prp-cache off) */
+};
+/* LDS-tier chain form (the reference's --shared-variant, cuckoo_bench.cu:527-563). */
+enum {
+    CK_LDS_AUTO = 0,  /* MAP */
+    CK_LDS_MAP = 1,   /* the build writes the bucket map into LDS; a probe reads it */
+    CK_LDS_ROUND = 2  /* no map: each probe evaluates its bucket over the LDS round cache */
+};
+
+/* Placement request of a context (the reference's --table-layout,
+ * --shared-variant, --prp-cache and --scratch-mib, cuckoo_bench.cu:406-419). */
+typedef struct ck_placement {
+    uint32_t tier;          /* CK_TIER_* */
+    uint32_t layout;        /* CK_LAYOUT_* (HBM tier) */
+    uint32_t prp;           /* CK_PRP_* */
+    uint32_t lds_variant;   /* CK_LDS_* */
+    uint64_t budget_bytes;  /* device memory for a launch's workspace (0: 95 % of the free memory) */
+} ck_placement;
+int ck_ctx_set_placement(ck_ctx* ctx, const ck_placement* want);
+
+/* What a launch of `count` trials would do. */
+typedef struct ck_plan {
+    uint32_t tier;             /* CK_TIER_HBM or CK_TIER_LDS */
+    uint32_t layout;           /* CK_LAYOUT_* chosen (HBM tier; CK_LAYOUT_AUTO for the LDS tier) */
+    uint32_t round_cache;      /* 1: bucket evaluations read a round cache; 0: direct AES */
+    uint32_t lds_variant;      /* CK_LDS_MAP or CK_LDS_ROUND (LDS tier) */
+    uint32_t trials_per_pass;  /* HBM: chains per pass; LDS: trials per fused launch */
+    uint32_t buffers;          /* HBM pass buffers */
+    uint32_t windows;          /* HBM item windows per pass (1: none) */
+    uint32_t chain_waves;      /* wave64s per chain (LDS: fused) block */
+    uint32_t chain_lds_bytes;  /* LDS per chain (LDS: fused) block */
+    uint32_t build_lds_bytes;  /* LDS per build block (HBM tier) */
+    uint64_t workspace_bytes;  /* device workspace of the launch */
+} ck_plan;
+/* The context's plan, with its device's free memory (no kernel runs). */
+int ck_ctx_plan(ck_ctx* ctx, const ck_params* p, uint32_t count, ck_plan* out);
+/* The same without a device: a budget of 0 plans against 96 GiB and 160 KB of
+ * LDS per block (MI355X). */
+int ck_plan_describe(const ck_params* p, const ck_placement* want, uint32_t count, ck_plan* out);
 
 /* One autotuner measurement. */
 typedef struct ck_tune_entry {

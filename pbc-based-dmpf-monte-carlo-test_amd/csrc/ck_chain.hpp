@@ -8,6 +8,7 @@
 #include <type_traits>
 
 #include "ck_device.hpp"
+#include "cuckoo_trial.hpp"
 
 namespace ck {
 
@@ -39,42 +40,21 @@ __device__ __forceinline__ constexpr E empty_entry() {
     return static_cast<E>(~0u);
 }
 
-// BGH25 random-walk insertion (cuckoo_core.hpp:123-133) with the k/stash
-// extension: each step draws which = r % K (refilling r from SplitMix64 when
-// r < K); an empty target takes the item; otherwise the walk fails (or stashes
-// the item in hand) once collisions exceed kick, else evicts the occupant.
+// The bucket-map layout runs the trial API of include/cuckoo_trial.hpp
+// directly: the slot's table as PackedBins of its entry width and its map rows
+// as PrecomputedBuckets.  (The other layouts below make the same decisions
+// over entries that carry the occupant's candidates.)
 template <int K, class M, class E>
 __device__ __forceinline__ bool chain_bgh(const Shape& S, const M* __restrict__ m, E* __restrict__ t, uint64_t walk,
                                           uint64_t rnd, uint32_t& probes) {
-    uint32_t item = 0, cur = 0, coll = 0, stash_left = S.stash, pr = 0;
-    bool failed = false;
-    while (item < S.pnum) {
-        if (rnd < K) rnd = splitmix(walk);
-        const uint32_t which = take_choice<K>(rnd);
-        const uint32_t b = m[cur * K + which];
-        const E old = t[b];
-        ++pr;
-        if (old == empty_entry<E>()) {
-            t[b] = pack_entry<E>(cur, which);
-            cur = ++item;
-            coll = 0;
-            continue;
-        }
-        if (++coll > S.kick) {
-            if (stash_left) {
-                --stash_left;
-                cur = ++item;
-                coll = 0;
-                continue;
-            }
-            failed = true;
-            break;
-        }
-        t[b] = pack_entry<E>(cur, which);
-        cur = entry_ordinal<E>(old);
-    }
-    probes = pr;
-    return failed;
+    return cuckoo::insert_core_k<true, K>(S.pnum, S.kick, S.stash, cuckoo::PackedBins<E, EntryCodec<E>::SHIFT>{t, 1},
+                                          cuckoo::PrecomputedBuckets<M, K>{m}, walk, rnd, probes);
+}
+template <int K, class M, class E>
+__device__ __forceinline__ bool chain_drrt(const Shape& S, const M* __restrict__ m, E* __restrict__ t,
+                                           uint32_t& probes) {
+    return cuckoo::insert_core_k<false, K>(S.pnum, S.kick, S.stash, cuckoo::PackedBins<E, EntryCodec<E>::SHIFT>{t, 1},
+                                           cuckoo::PrecomputedBuckets<M, K>{m}, 0, 0, probes);
 }
 
 // An item's K <= 4 bucket indices, as one vector value.  A plain array would be
@@ -93,80 +73,6 @@ __device__ __forceinline__ uint32_t pick(const Cands& c, uint32_t i) {
     if constexpr (K > 2) v = (i == 2u) ? c.z : v;
     if constexpr (K > 3) v = (i == 3u) ? c.w : v;
     return v;
-}
-
-// DRRT18 deterministic insertion (cuckoo_core.hpp:134-158) with the k/stash
-// extension: try candidates 0..K-1 in order; otherwise evict from candidate 0
-// and move the displaced item to its next candidate (stored choice + 1, mod K)
-// whose bucket differs from the one it left; fail (or stash) at kick evictions
-// or when no such candidate exists.
-template <int K, class M, class E>
-__device__ __forceinline__ bool chain_drrt(const Shape& S, const M* __restrict__ m, E* __restrict__ t,
-                                           uint32_t& probes) {
-    uint32_t pr = 0, stash_left = S.stash;
-    bool failed = false;
-    for (uint32_t item = 0; item < S.pnum && !failed; ++item) {
-        Cands c = {};
-#pragma unroll
-        for (int w = 0; w < K; ++w) c[w] = m[item * K + w];
-        bool placed = false;
-#pragma unroll
-        for (int w = 0; w < K; ++w) {
-            if (placed) break;
-            ++pr;
-            if (t[c[w]] == empty_entry<E>()) {
-                t[c[w]] = pack_entry<E>(item, uint32_t(w));
-                placed = true;
-            }
-        }
-        if (placed) continue;
-        uint32_t cur = item, choice = 0, target = c[0], kicks = 0;
-        for (;;) {
-            const E disp = t[target];
-            ++pr;
-            if (disp == empty_entry<E>()) {
-                t[target] = pack_entry<E>(cur, choice);
-                break;
-            }
-            if (kicks >= S.kick) {
-                if (stash_left) {
-                    --stash_left;
-                    break;
-                }
-                failed = true;
-                break;
-            }
-            t[target] = pack_entry<E>(cur, choice);
-            cur = entry_ordinal<E>(disp);
-            choice = entry_choice<E>(disp);
-            const uint32_t prev = target;
-            Cands cc = {};
-#pragma unroll
-            for (int w = 0; w < K; ++w) cc[w] = m[cur * K + w];
-            bool found = false;
-#pragma unroll
-            for (int s = 0; s < K; ++s) {
-                if (found) break;
-                choice = (choice + 1 == K) ? 0 : choice + 1;
-                const uint32_t nb = pick<K>(cc, choice);
-                if (nb != prev) {
-                    target = nb;
-                    found = true;
-                }
-            }
-            if (!found) {
-                if (stash_left) {
-                    --stash_left;
-                    break;
-                }
-                failed = true;
-                break;
-            }
-            ++kicks;
-        }
-    }
-    probes = pr;
-    return failed;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,7 +345,7 @@ struct ItemSpan {
     uint32_t begin, end;
 };
 
-// BGH25 over the residual layout (same decisions as chain_bgh).
+// BGH25 over the residual layout (the decisions of cuckoo::insert_core_k).
 // T is the table: an E* or Entries48 (E = u64).
 template <int K, class E, class M, class T>
 __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
@@ -482,7 +388,7 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
     return failed;
 }
 
-// DRRT18 over the residual layout (same decisions as chain_drrt).
+// DRRT18 over the residual layout (the decisions of cuckoo::insert_core_k).
 template <int K, class E, class M, class T>
 __device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
                                                T t, ItemSpan span, uint32_t& stash_left, uint32_t& probes) {
@@ -601,7 +507,7 @@ struct DescView {
     __device__ __forceinline__ uint64_t item(uint32_t o) const { return desc[o]; }
 };
 
-// BGH25 with exchange probes (same decisions as chain_bgh).
+// BGH25 with exchange probes (the decisions of cuckoo::insert_core_k).
 template <int K, class E, class V>
 __device__ __forceinline__ bool chain_bgh_xchg(const Shape& S, const V& v, E* __restrict__ t, uint64_t walk,
                                                uint64_t rnd, uint32_t& probes) {

@@ -160,25 +160,6 @@ __device__ __forceinline__ uint64_t element_of(const Shape& S, uint64_t shift, u
     return S.input_mode == 0 ? uint64_t(ordinal) : (shift + uint64_t(ordinal) * step) & S.range_mask;
 }
 
-// Feistel PRP over the round cache (half <= 16, so y and the domain fit 32 bits):
-// 8 rounds L,R <- R, L ^ F_r(R), cycle-walked into [0, domain), then / bucket_size.
-__device__ __forceinline__ uint32_t feistel_cached(const uint16_t* cache, const Shape& S, uint64_t elem, uint32_t k,
-                                                   uint32_t which) {
-    const uint32_t h = S.half, m = static_cast<uint32_t>(S.mask);
-    uint32_t y = static_cast<uint32_t>(elem * k + which);
-    do {
-        uint32_t L = y >> h, R = y & m;
-#pragma unroll
-        for (uint32_t r = 0; r < 8; ++r) {
-            const uint32_t nx = L ^ cache[(r << h) + R];
-            L = R;
-            R = nx & m;
-        }
-        y = (L << h) | R;
-    } while (uint64_t(y) >= S.domain);
-    return div_bucket(S, y);
-}
-
 // N independent AES-128 blocks advanced round by round together (more loads in
 // flight per lane); returns the last output word of each block.
 template <int N>
@@ -215,7 +196,8 @@ __device__ __forceinline__ void aes_last_n(const LdsTab& T, const uint32_t* rk, 
 // lockstep: each pass applies the 8 rounds to every candidate (the k dependent
 // LDS chains overlap) and keeps the result only for candidates still outside
 // the domain, so each candidate ends after exactly the passes its own cycle
-// walk takes.  Same values as k calls of feistel_cached.
+// walk takes.  Same values as k evaluations of cuckoo::CachedBuckets
+// (include/cuckoo_trial.hpp) over the same cache.
 template <int K>
 __device__ __forceinline__ void feistel_cached_k(const uint16_t* cache, const Shape& S, uint64_t elem,
                                                  uint32_t (&out)[K]) {

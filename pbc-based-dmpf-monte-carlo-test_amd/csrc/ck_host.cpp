@@ -212,4 +212,46 @@ void ck_host_aes(const uint32_t round_keys[44], const uint32_t in[4], uint32_t o
     ck::host_aes(round_keys, in, out);
 }
 const char* ck_last_error(void) { return ck::last_error(); }
+
+int ck_trial_inputs_of(const ck_params* p, const uint32_t batch_round_keys[44], uint32_t offset,
+                       ck_trial_inputs* out) {
+    const int rc = ck::check_params(p);
+    if (rc) return rc;
+    if (!batch_round_keys || !out) return ck::fail(CK_EINVAL, "null argument");
+    const auto splitmix = [](uint64_t& state) {
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    uint32_t w[4];
+    const uint32_t key_block[4] = {0x4b455931u, 0u, offset, 0x50525031u};  // "KEY1", 0, offset, "PRP1"
+    ck::host_aes(batch_round_keys, key_block, w);
+    ck::expand_words(w, out->prp_key);
+    const uint32_t walk_block[4] = {0x57414c4bu, 0u, offset, 0x524e4731u};  // "WALK", 0, offset, "RNG1"
+    ck::host_aes(batch_round_keys, walk_block, w);
+    out->walk = (uint64_t(w[0]) << 32) | w[1];
+    out->shift = ((uint64_t(w[2]) << 32) | w[3]) & p->range_mask;
+    uint64_t state = out->shift ^ 0xd1b54a32d192ed03ull;
+    out->step = (splitmix(state) | 1ull) & p->range_mask;
+    out->randomness = splitmix(out->walk);
+    return CK_OK;
+}
+
+int ck_round_cache(const ck_params* p, const uint32_t prp_key[44], uint16_t* rounds) {
+    const int rc = ck::check_params(p);
+    if (rc) return rc;
+    if (!prp_key || !rounds) return ck::fail(CK_EINVAL, "null argument");
+    if (p->half_bits > 16) return ck::fail(CK_EINVAL, "a round cache needs half_bits <= 16");
+    const uint32_t half = p->half_bits, n = 1u << half;
+    for (uint32_t r = 0; r < 8; ++r)
+        for (uint32_t right = 0; right < n; ++right) {
+            const uint32_t in[4] = {0x43455350u, r, half, right};  // "CESP", round, half, right
+            uint32_t o[4];
+            ck::host_aes(prp_key, in, o);
+            rounds[(r << half) | right] = static_cast<uint16_t>(o[3] & p->mask);
+        }
+    return CK_OK;
+}
 }

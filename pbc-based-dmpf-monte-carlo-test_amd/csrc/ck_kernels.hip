@@ -254,8 +254,12 @@ __global__ __launch_bounds__(256) void k_digest_res(Shape S, uint32_t count, con
 }
 
 // LDS tier: a persistent wave per trial at a time.  Wave region layout:
-// [round cache 8<<half u16][map pnum*K M][table buckets E].
-template <int K, bool BGH, class M, class E>
+// [round cache 8<<half u16][map pnum*K M, unless ROUND][table buckets E].
+// The chain is the trial API's insert_core_k (include/cuckoo_trial.hpp) on
+// lane 0, its buckets read from the map (PrecomputedBuckets) or, with ROUND,
+// evaluated over the round cache at every probe (CachedBuckets: no map, so a
+// trial needs less LDS).
+template <int K, bool BGH, class M, class E, bool ROUND>
 __global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t first, uint32_t count, uint32_t tab_shift,
                                              uint32_t wave_bytes, uint32_t cache_bytes, uint32_t map_bytes,
                                              uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
@@ -268,6 +272,7 @@ __global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t fir
     uint16_t* cache = reinterpret_cast<uint16_t*>(region);
     M* map = reinterpret_cast<M*>(region + cache_bytes);
     E* table = reinterpret_cast<E*>(region + cache_bytes + map_bytes);
+    const cuckoo::PackedBins<E, EntryCodec<E>::SHIFT> bins{table, 1};
     // Every iteration starts and ends wave-converged: the only lane-divergent
     // region is the chain itself.  Single-lane side effects (claiming a slot,
     // publishing the outcome) are written as whole-wave operations on
@@ -281,21 +286,28 @@ __global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t fir
         TrialSeed ts;
         derive_uniform(T, bk, S, first + slot, ts);
         build_cache(T, ts, S, cache, lane);
-        for (uint32_t o = lane; o < S.pnum; o += 64) {
-            uint32_t c[K];
-            feistel_cached_k<K>(cache, S, element_of(S, ts.shift, ts.step, o), c);
+        const cuckoo::CachedBuckets<uint16_t, K> feistel{cache, 1, S.domain, S.bucket_size, S.mask, S.range_mask,
+                                                         ts.shift, ts.step, S.half, S.input_mode == 0};
+        if constexpr (!ROUND) {
+            for (uint32_t o = lane; o < S.pnum; o += 64) {
+                uint32_t c[K];
+                feistel_cached_k<K>(cache, S, element_of(S, ts.shift, ts.step, o), c);
 #pragma unroll
-            for (int w = 0; w < K; ++w) map[o * K + w] = static_cast<M>(c[w]);
+                for (int w = 0; w < K; ++w) map[o * K + w] = static_cast<M>(c[w]);
+            }
         }
         for (uint32_t i = lane; i < S.buckets; i += 64) table[i] = empty_entry<E>();
         wave_lds_sync();
         bool failed_lane = false;
         uint32_t pr_lane = 0;
         if (lane == 0) {
-            if constexpr (BGH)
-                failed_lane = chain_bgh<K>(S, map, table, ts.walk, ts.rnd, pr_lane);
+            if constexpr (ROUND)
+                failed_lane = cuckoo::insert_core_k<BGH, K>(S.pnum, S.kick, S.stash, bins, feistel, ts.walk, ts.rnd,
+                                                            pr_lane);
             else
-                failed_lane = chain_drrt<K>(S, map, table, pr_lane);
+                failed_lane = cuckoo::insert_core_k<BGH, K>(S.pnum, S.kick, S.stash, bins,
+                                                            cuckoo::PrecomputedBuckets<M, K>{map}, ts.walk, ts.rnd,
+                                                            pr_lane);
         }
         wave_lds_sync();
         const uint32_t failed = __builtin_amdgcn_readfirstlane(failed_lane ? 1u : 0u);
@@ -315,7 +327,8 @@ __global__ __launch_bounds__(512) void k_lds(Shape S, RoundKeys bk, uint32_t fir
                 const uint32_t o = entry_ordinal<E>(e);
                 uint64_t h = 0;
 #pragma unroll
-                for (int w = 0; w < K; ++w) h += cand_hash_term(w, map[o * K + w]);
+                for (int w = 0; w < K; ++w)
+                    h += cand_hash_term(w, ROUND ? feistel(o, uint32_t(w)) : uint32_t(map[o * K + w]));
                 part += slot_digest(b, h, entry_choice<E>(e));
             }
 #pragma unroll
@@ -442,6 +455,9 @@ struct Ctx {
     uint32_t layout = CK_LAYOUT_AUTO;
     uint32_t chunk_cap = 0;
     uint32_t window = 0;  // items per residual-layout window (0 = automatic)
+    uint32_t prp = CK_PRP_AUTO;          // round cache or direct AES for bucket evaluations
+    uint32_t lds_variant = CK_LDS_AUTO;  // LDS tier: bucket map or round cache per probe
+    size_t budget_fixed = 0;             // workspace budget set by the caller (0: from the free memory)
     size_t budget = size_t(96) << 30;  // HBM budget for the double-buffered maps and tables (set per call)
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
@@ -482,11 +498,14 @@ struct BuildPlan {
     uint32_t waves;  // trials per build block
     size_t lds;
 };
-static BuildPlan plan_build(const ck_params& p, size_t lds_block) {
+static bool cache_possible(const ck_params& p, size_t lds_block) {
+    return p.half_bits <= kMaxCacheHalf && tab_bytes(kRepl) + (size_t(8) << p.half_bits) * sizeof(uint16_t) <= lds_block;
+}
+static BuildPlan plan_build(const ck_params& p, size_t lds_block, uint32_t prp) {
     BuildPlan b{};
     const size_t tab = tab_bytes(kRepl);
     const size_t cache = (size_t(8) << p.half_bits) * sizeof(uint16_t);
-    b.cached = p.half_bits <= kMaxCacheHalf && tab + cache <= lds_block;
+    b.cached = prp != CK_PRP_DIRECT && cache_possible(p, lds_block);
     if (b.cached) {
         b.waves = static_cast<uint32_t>(std::min<size_t>(kBuildWaves, (lds_block - tab) / cache));
         b.lds = tab + b.waves * cache;
@@ -498,18 +517,19 @@ static BuildPlan plan_build(const ck_params& p, size_t lds_block) {
 }
 
 struct LdsPlan {
-    bool ok = false, map16 = false, bins16 = false;
+    bool ok = false, map16 = false, bins16 = false, round = false;
     uint32_t waves = 0, tab_shift = 0, blocks_per_cu = 0;
     size_t cache_b = 0, map_b = 0, bins_b = 0, wave_b = 0, lds = 0;
 };
-static LdsPlan plan_lds(const ck_params& p, size_t lds_block, size_t lds_cu) {
+static LdsPlan plan_lds(const ck_params& p, size_t lds_block, size_t lds_cu, bool round) {
     LdsPlan L;
+    L.round = round;
     if (p.half_bits > kMaxCacheHalf) return L;
     auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
     L.map16 = map16_ok(p);
     L.bins16 = table16_ok(p);
     L.cache_b = a16((size_t(8) << p.half_bits) * 2);
-    L.map_b = a16(size_t(p.pnum) * p.k * (L.map16 ? 2 : 4));
+    L.map_b = round ? 0 : a16(size_t(p.pnum) * p.k * (L.map16 ? 2 : 4));
     L.bins_b = a16(size_t(p.buckets) * (L.bins16 ? 2 : 4));
     L.wave_b = L.cache_b + L.map_b + L.bins_b;
     uint32_t best = 0;
@@ -671,6 +691,7 @@ static uint32_t stream_window(const Ctx& c, const ck_params& p, uint32_t count, 
 // latency-bound, so every extra slot is an extra chain in flight.
 static constexpr size_t kBudgetCap = size_t(272) << 30;
 static size_t hbm_budget(const Ctx& c) {
+    if (c.budget_fixed) return c.budget_fixed;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return size_t(96) << 30;
     return std::min(kBudgetCap, (free_b + c.ws.bytes) / 20 * 19);
@@ -707,13 +728,24 @@ static void forget_tables(Ctx& c) {
     c.table_tags[0] = c.table_tags[1] = {};
 }
 
+// Grows the workspace.  Graphs whose kernel nodes point into the old
+// workspace are destroyed first (they could never be replayed against the new
+// one).  CK_ENOMEM when the device cannot hold `bytes`.
 static int ensure_ws(Ctx& c, size_t bytes) {
     if (c.ws.bytes >= bytes) return CK_OK;
     forget_tables(c);
+    for (HbmGraph& g : c.graphs)
+        if (g.exec && g.ws == c.ws.base) g.drop();
     if (c.ws.base) CKH(hipFree(c.ws.base));
     c.ws.base = nullptr;
     c.ws.bytes = 0;
-    CKH(hipMalloc(&c.ws.base, bytes));
+    const hipError_t e = hipMalloc(&c.ws.base, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        c.ws.base = nullptr;
+        return fail(CK_ENOMEM, "device memory: cannot allocate a " + std::to_string(bytes >> 20) + " MiB workspace");
+    }
+    if (e != hipSuccess) return fail(CK_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
     c.ws.bytes = bytes;
     return CK_OK;
 }
@@ -863,14 +895,18 @@ static void launch_lds_kb(const LdsPlan& L, uint32_t blocks, const Shape& S, con
     const uint32_t wb = static_cast<uint32_t>(L.wave_b), cb = static_cast<uint32_t>(L.cache_b),
                    mb = static_cast<uint32_t>(L.map_b);
 #define LDS_ARGS S, bk, first, count, L.tab_shift, wb, cb, mb, cnt, fo, probes, tables, digests
-    if (L.map16 && L.bins16)
-        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint16_t>), grid, block, L.lds, st, LDS_ARGS);
+    if (L.round && L.bins16)
+        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint16_t, true>), grid, block, L.lds, st, LDS_ARGS);
+    else if (L.round)
+        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint32_t, true>), grid, block, L.lds, st, LDS_ARGS);
+    else if (L.map16 && L.bins16)
+        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint16_t, false>), grid, block, L.lds, st, LDS_ARGS);
     else if (L.map16)
-        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint32_t>), grid, block, L.lds, st, LDS_ARGS);
+        hipLaunchKernelGGL((k_lds<K, BGH, uint16_t, uint32_t, false>), grid, block, L.lds, st, LDS_ARGS);
     else if (L.bins16)
-        hipLaunchKernelGGL((k_lds<K, BGH, uint32_t, uint16_t>), grid, block, L.lds, st, LDS_ARGS);
+        hipLaunchKernelGGL((k_lds<K, BGH, uint32_t, uint16_t, false>), grid, block, L.lds, st, LDS_ARGS);
     else
-        hipLaunchKernelGGL((k_lds<K, BGH, uint32_t, uint32_t>), grid, block, L.lds, st, LDS_ARGS);
+        hipLaunchKernelGGL((k_lds<K, BGH, uint32_t, uint32_t, false>), grid, block, L.lds, st, LDS_ARGS);
 #undef LDS_ARGS
 }
 template <int K>
@@ -885,12 +921,14 @@ static void launch_lds(bool bgh, const LdsPlan& L, uint32_t blocks, const Shape&
 
 template <class Fn>
 static int for_each_lds_kernel(Fn&& fn) {
-#define ONE(K, B, M, E)                                                              \
+#define ONE(K, B, M, E, R)                                                           \
     {                                                                                \
-        const int rc = fn(reinterpret_cast<const void*>(&k_lds<K, B, M, E>));        \
+        const int rc = fn(reinterpret_cast<const void*>(&k_lds<K, B, M, E, R>));     \
         if (rc) return rc;                                                           \
     }
-#define ALL_ME(K, B) ONE(K, B, uint16_t, uint16_t) ONE(K, B, uint16_t, uint32_t) ONE(K, B, uint32_t, uint16_t) ONE(K, B, uint32_t, uint32_t)
+#define ALL_ME(K, B)                                                                                   \
+    ONE(K, B, uint16_t, uint16_t, false) ONE(K, B, uint16_t, uint32_t, false) ONE(K, B, uint32_t, uint16_t, false) \
+    ONE(K, B, uint32_t, uint32_t, false) ONE(K, B, uint16_t, uint16_t, true) ONE(K, B, uint16_t, uint32_t, true)
     ALL_ME(2, true) ALL_ME(2, false) ALL_ME(3, true) ALL_ME(3, false) ALL_ME(4, true) ALL_ME(4, false)
 #undef ALL_ME
 #undef ONE
@@ -1114,11 +1152,21 @@ static int run_hbm_graph(Ctx& c, const ck_params& p, uint32_t first, uint32_t co
     return CK_OK;
 }
 
-// HBM tier: build + chain per pass of up to `chunk` trial slots.  Two buffers
-// and two streams let the build of pass i+1 run while pass i chains.
-static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
-                   ck_stats* st) {
-    c.budget = hbm_budget(c);
+// What an HBM-tier launch of `count` trials will use: the table layout, item
+// windows, pass buffers, trials per pass, the workspace plan and the build.
+struct HbmDecision {
+    uint32_t layout = 0, window = 0, nbuf = 0, chunk = 0;
+    bool graph = false;
+    HbmPlan h{};
+    BuildPlan bp{};
+};
+// `tables`: the caller wants final tables (ordinal entries); `graph`: the launch
+// runs as a graph; `windows_ok`: item windows are allowed (no per-trial map,
+// walk or random-word output).  Uses c.budget.
+static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tables, bool graph, bool windows_ok,
+                      HbmDecision& d) {
+    if (c.prp == CK_PRP_CACHE && !cache_possible(p, c.lds_block))
+        return fail(CK_EINVAL, "a round cache needs half_bits <= 12 (this shape evaluates AES directly)");
     // Auto: descriptor entries with load + store probes where they fit (fastest
     // with tens of thousands of chains in flight, and reusable without a clear);
     // otherwise residual entries, still one random read per probe (measured
@@ -1128,9 +1176,9 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     // trip per probe matters when latency, not throughput, bounds the pass).
     uint32_t layout = c.layout;
     if (layout == CK_LAYOUT_AUTO) {
-        if (!out.tables && desc_ok(p))
+        if (!tables && desc_ok(p))
             layout = CK_LAYOUT_DESC_PLAIN;
-        else if (!out.tables && res_ok(p))
+        else if (!tables && res_ok(p))
             layout = CK_LAYOUT_RESIDUAL;
         else
             layout = hbm_chunk(c, p, count, CK_LAYOUT_MAP_XCHG, hbm_buffers(c, p, count, CK_LAYOUT_MAP_XCHG),
@@ -1140,20 +1188,45 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     }
     const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     if (desc && !desc_ok(p)) return fail(CK_EINVAL, "descriptor layout needs k*bits(buckets-1) + 2 <= 63");
-    if (desc && out.tables) return fail(CK_EINVAL, "the descriptor layout does not keep ordinals; use the map layout for tables");
+    if (desc && tables) return fail(CK_EINVAL, "the descriptor layout does not keep ordinals; use the map layout for tables");
     if (layout == CK_LAYOUT_RESIDUAL && !res_ok(p))
         return fail(CK_EINVAL, "residual layout needs (k-1)*bits(buckets-1) + 2 <= 63");
-    if (layout == CK_LAYOUT_RESIDUAL && out.tables)
+    if (layout == CK_LAYOUT_RESIDUAL && tables)
         return fail(CK_EINVAL, "the residual layout does not keep ordinals; use the map layout for tables");
-    const bool graph =
+    d.layout = layout;
+    d.graph = graph;
+    d.window = stream_window(c, p, count, layout, !graph && windows_ok);
+    d.nbuf = d.window < p.pnum ? 1 : hbm_buffers(c, p, count, layout);
+    d.chunk = hbm_chunk(c, p, count, layout, d.nbuf, d.window);
+    d.h = plan_hbm(p, d.chunk, layout, d.window);
+    d.bp = plan_build(p, c.lds_block, c.prp);
+    return CK_OK;
+}
+
+// HBM tier: build + chain per pass of up to `chunk` trial slots.  Two buffers
+// and two streams let the build of pass i+1 run while pass i chains.
+static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
+                   ck_stats* st) {
+    c.budget = hbm_budget(c);
+    const bool graph_wanted =
         c.graph_on && !out.probes && !out.tables && !out.keys && !out.walks && !out.rnds && !out.map && !out.digests;
-    const uint32_t window = stream_window(c, p, count, layout, !graph && !out.map && !out.walks && !out.rnds);
-    const uint32_t nbuf = window < p.pnum ? 1 : hbm_buffers(c, p, count, layout);
-    const uint32_t chunk = hbm_chunk(c, p, count, layout, nbuf, window);
-    const HbmPlan h = plan_hbm(p, chunk, layout, window);
-    int rc = ensure_ws(c, nbuf * h.buffer);
+    HbmDecision d;
+    int rc = CK_OK;
+    // A workspace that does not fit (another context on the device took memory
+    // since the budget was read) is planned again at half the budget.
+    for (int attempt = 0;; ++attempt) {
+        rc = decide_hbm(c, p, count, out.tables != nullptr, graph_wanted, !out.map && !out.walks && !out.rnds, d);
+        if (rc) return rc;
+        rc = ensure_ws(c, d.nbuf * d.h.buffer);
+        if (rc != CK_ENOMEM || attempt == 8 || d.chunk == 1) break;
+        c.budget /= 2;
+    }
     if (rc) return rc;
-    const BuildPlan bp = plan_build(p, c.lds_block);
+    const uint32_t layout = d.layout, window = d.window, nbuf = d.nbuf, chunk = d.chunk;
+    const bool graph = d.graph;
+    const HbmPlan& h = d.h;
+    const BuildPlan& bp = d.bp;
+    (void)window;
     const bool bgh = p.mode == CK_MODE_BGH25;
     PassArgs args[2];
     for (uint32_t b = 0; b < nbuf; ++b) {
@@ -1348,7 +1421,9 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
 // LDS tier: one fused persistent launch per chunk.
 static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
                    ck_stats* st) {
-    const LdsPlan plan = plan_lds(p, c.lds_block, c.lds_cu);
+    if (c.prp == CK_PRP_DIRECT)
+        return fail(CK_EINVAL, "the LDS tier builds a round cache: it cannot run with direct AES evaluation");
+    const LdsPlan plan = plan_lds(p, c.lds_block, c.lds_cu, c.lds_variant == CK_LDS_ROUND);
     if (!plan.ok) return fail(CK_EINVAL, "shape does not fit the LDS tier (half_bits > 12 or table too large)");
     forget_tables(c);  // the workspace becomes this tier's scratch
     const Shape S = make_shape(p);
@@ -1401,6 +1476,56 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         }
         done += n;
     }
+    return CK_OK;
+}
+
+// The plan of a launch of `count` trials under the context's settings
+// (c.budget must be set); no device work.
+static int plan_launch(const Ctx& c, const ck_params& p, uint32_t count, ck_plan& out) {
+    std::memset(&out, 0, sizeof out);
+    if (choose_tier(c, p) == CK_TIER_LDS) {
+        if (c.prp == CK_PRP_DIRECT)
+            return fail(CK_EINVAL, "the LDS tier builds a round cache: it cannot run with direct AES evaluation");
+        const LdsPlan L = plan_lds(p, c.lds_block, c.lds_cu, c.lds_variant == CK_LDS_ROUND);
+        if (!L.ok) return fail(CK_EINVAL, "shape does not fit the LDS tier (half_bits > 12 or table too large)");
+        out.tier = CK_TIER_LDS;
+        out.layout = CK_LAYOUT_AUTO;
+        out.round_cache = 1;
+        out.lds_variant = L.round ? CK_LDS_ROUND : CK_LDS_MAP;
+        out.trials_per_pass = lds_chunk(c, std::max<uint32_t>(count, 1), false, p);
+        out.buffers = 1;
+        out.windows = 1;
+        out.chain_waves = L.waves;
+        out.chain_lds_bytes = static_cast<uint32_t>(L.lds);
+        out.workspace_bytes = lds_scratch(p, out.trials_per_pass, false).total;
+        return CK_OK;
+    }
+    HbmDecision d;
+    const int rc = decide_hbm(c, p, std::max<uint32_t>(count, 1), false, c.graph_on, true, d);
+    if (rc) return rc;
+    out.tier = CK_TIER_HBM;
+    out.layout = d.layout;
+    out.round_cache = d.bp.cached ? 1 : 0;
+    out.lds_variant = CK_LDS_AUTO;
+    out.trials_per_pass = d.chunk;
+    out.buffers = d.nbuf;
+    out.windows = d.h.nwin;
+    out.chain_waves = 4;  // k_chain blocks are 256 lanes
+    out.chain_lds_bytes = 0;
+    out.build_lds_bytes = static_cast<uint32_t>(d.bp.lds);
+    out.workspace_bytes = uint64_t(d.nbuf) * d.h.buffer;
+    return CK_OK;
+}
+
+static int apply_placement(Ctx& c, const ck_placement* w) {
+    if (!w || w->tier > CK_TIER_LDS || w->layout > CK_LAYOUT_RESIDUAL || w->prp > CK_PRP_DIRECT ||
+        w->lds_variant > CK_LDS_ROUND)
+        return fail(CK_EINVAL, "bad placement");
+    c.tier = w->tier;
+    c.layout = w->layout;
+    c.prp = w->prp;
+    c.lds_variant = w->lds_variant;
+    c.budget_fixed = static_cast<size_t>(w->budget_bytes);
     return CK_OK;
 }
 
@@ -1593,7 +1718,8 @@ int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t
         cands.push_back({CK_TIER_HBM, layout, 0});
         if (half < sample) cands.push_back({CK_TIER_HBM, layout, half});
     }
-    if (plan_lds(*p, c.lds_block, c.lds_cu).ok) cands.push_back({CK_TIER_LDS, CK_LAYOUT_AUTO, 0});
+    if (c.prp != CK_PRP_DIRECT && plan_lds(*p, c.lds_block, c.lds_cu, c.lds_variant == CK_LDS_ROUND).ok)
+        cands.push_back({CK_TIER_LDS, CK_LAYOUT_AUTO, 0});
     const uint32_t saved_tier = c.tier, saved_layout = c.layout, saved_chunk = c.chunk_cap;
     std::vector<uint32_t> want, got(sample);
     bool have_want = false;
@@ -1631,6 +1757,32 @@ int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t
     c.layout = rc == CK_OK ? best.layout : saved_layout;
     c.chunk_cap = rc == CK_OK ? best.chunk : saved_chunk;
     return rc;
+}
+
+int ck_ctx_set_placement(ck_ctx* ctx, const ck_placement* want) {
+    if (!ctx) return fail(CK_EINVAL, "null ctx");
+    return apply_placement(ctx->c, want);
+}
+
+int ck_ctx_plan(ck_ctx* ctx, const ck_params* p, uint32_t count, ck_plan* out) {
+    if (!ctx || !out) return fail(CK_EINVAL, "null argument");
+    int rc = check_params(p);
+    if (rc) return rc;
+    CKH(hipSetDevice(ctx->c.device));
+    ctx->c.budget = hbm_budget(ctx->c);
+    return plan_launch(ctx->c, *p, count, *out);
+}
+
+int ck_plan_describe(const ck_params* p, const ck_placement* want, uint32_t count, ck_plan* out) {
+    if (!out) return fail(CK_EINVAL, "null output");
+    int rc = check_params(p);
+    if (rc) return rc;
+    Ctx c;  // settings only: no streams, no memory
+    if ((rc = apply_placement(c, want))) return rc;
+    c.lds_block = c.lds_cu = size_t(160) << 10;
+    c.cus = 256;
+    c.budget = c.budget_fixed ? c.budget_fixed : size_t(96) << 30;
+    return plan_launch(c, *p, count, *out);
 }
 
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout) {

@@ -1,195 +1,315 @@
-// cuckoo_bench.cpp -- the cuckoo_bench command line (cuckoo_bench.cu:1062-1142)
-// on the MI355X library: resume from the journals, one worker thread per
-// context, batches claimed through a shared atomic cursor, each batch run by
-// process_batch over ck_launch and committed durably, then summary.json,
-// failures.csv and the RESULT line.
+// cuckoo_bench.cpp -- the cuckoo_bench command line (the reference's
+// cuckoo_bench.cu:1062-1142 interface: same flags, journals, summary.json and
+// failures.csv) on the MI355X library.
+//
+// A run: lock the output directory, rebuild progress from the journals, then
+// start one worker thread per context; workers claim batch ids from a shared
+// cursor, skip committed ones, and run each batch through process_batch over
+// ck_launch, committing it to their own journal.  When they stop (plan done,
+// failure threshold, stop request or an error) the journals are scanned again
+// and the reports are written from that scan.
 #include <hip/hip_runtime_api.h>
 
-#include <iomanip>
+#include <chrono>
+#include <cstdio>
+#include <exception>
 #include <iostream>
 #include <memory>
 #include <mutex>
 #include <thread>
 
-#include "ck_journal.hpp"
+#include "ck_ledger.hpp"
 
-using namespace ckh;
+using namespace ckrun;
 
 namespace {
 
-void check(int rc, const char* what) {
-    if (rc != CK_OK) throw std::runtime_error(std::string(what) + ": " + ck_last_error());
+void check(int rc, const char* call) {
+    if (rc != CK_OK) throw std::runtime_error(std::string(call) + ": " + ck_last_error());
 }
 
-struct Ctx {
-    ck_ctx* h = nullptr;
-    explicit Ctx(int device) { check(ck_ctx_create(device, &h), "ck_ctx_create"); }
-    ~Ctx() { ck_ctx_destroy(h); }
-    Ctx(const Ctx&) = delete;
+// The library placement a spec asks for.  --table-layout shared is the LDS
+// tier (its --shared-variant picks the map or the round-cache form);
+// --table-layout global and --global-table the HBM tier; otherwise --tier.
+ck_placement placement_of(const RunSpec& s, uint64_t budget_bytes) {
+    ck_placement w{};
+    if (s.table_layout == "shared")
+        w.tier = CK_TIER_LDS;
+    else if (s.table_layout == "global" || s.global_table)
+        w.tier = CK_TIER_HBM;
+    else
+        w.tier = s.tier == "hbm" ? CK_TIER_HBM : s.tier == "lds" ? CK_TIER_LDS : CK_TIER_AUTO;
+    w.layout = CK_LAYOUT_AUTO;
+    w.prp = s.prp_cache == "on" ? CK_PRP_CACHE : s.prp_cache == "off" ? CK_PRP_DIRECT : CK_PRP_AUTO;
+    w.lds_variant = s.shared_variant == "round" ? CK_LDS_ROUND : s.shared_variant == "map" ? CK_LDS_MAP : CK_LDS_AUTO;
+    w.budget_bytes = budget_bytes;
+    return w;
+}
+
+const char* layout_name(uint32_t layout) {
+    switch (layout) {
+        case CK_LAYOUT_MAP: return "map";
+        case CK_LAYOUT_DESC: return "desc";
+        case CK_LAYOUT_MAP_XCHG: return "map-xchg";
+        case CK_LAYOUT_DESC_PLAIN: return "desc-plain";
+        case CK_LAYOUT_RESIDUAL: return "residual";
+        default: return "auto";
+    }
+}
+
+// The planned placement in the reference's vocabulary where one exists
+// (shared-map / shared-trial for workgroup-LDS tables, cuckoo_bench.cu:527-563)
+// and <cached|direct>-global-<table layout> for the HBM tier.
+std::string placement_name(const ck_plan& pl) {
+    if (pl.tier == CK_TIER_LDS) return pl.lds_variant == CK_LDS_ROUND ? "shared-trial" : "shared-map";
+    return std::string(pl.round_cache ? "cached" : "direct") + "-global-" + layout_name(pl.layout);
+}
+
+class Context {
+   public:
+    explicit Context(int device) { check(ck_ctx_create(device, &h_), "ck_ctx_create"); }
+    ~Context() { ck_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    ck_ctx* get() const { return h_; }
+
+   private:
+    ck_ctx* h_ = nullptr;
 };
 
-uint32_t tier_code(const std::string& t) { return t == "hbm" ? CK_TIER_HBM : t == "lds" ? CK_TIER_LDS : CK_TIER_AUTO; }
+// Everything the workers share.
+struct Shared {
+    RunSpec spec;
+    ck_params shape{};
+    std::vector<uint8_t> committed;  // batches already in the journals at start
+    std::vector<uint64_t> budgets;   // device memory per worker, by device (0: library decides)
+    RunTally tally;
+    std::mutex log;
+    std::vector<std::exception_ptr> errors;
+};
 
-void worker(int id, int device, const Options& o, const ck_params& p, const std::vector<uint8_t>& done, Counters& s,
-            std::mutex& print_mutex) {
-    Ctx ctx(device);
-    check(ck_ctx_set_tier(ctx.h, tier_code(o.tier)), "ck_ctx_set_tier");
-    check(ck_ctx_set_graph(ctx.h, o.graph ? 1 : 0), "ck_ctx_set_graph");
-    const uint64_t batches = total_batches(o);
-    const uint32_t micro = static_cast<uint32_t>(std::min<uint64_t>(o.micro, 1ULL << o.bt));
+void worker_main(Shared& sh, int index, int device) {
+    const RunSpec& s = sh.spec;
+    Context ctx(device);
+    const ck_placement want = placement_of(s, sh.budgets[device]);
+    check(ck_ctx_set_placement(ctx.get(), &want), "ck_ctx_set_placement");
+    check(ck_ctx_set_graph(ctx.get(), s.graph ? 1 : 0), "ck_ctx_set_graph");
+    const uint32_t micro = static_cast<uint32_t>(std::min<uint64_t>(s.micro, uint64_t(1) << s.bt));
     std::string tuned;
-    if (o.tune) {
-        // A sample of batch 0 under every placement; results are not committed.
-        const uint32_t sample = static_cast<uint32_t>(std::min<uint64_t>(batch_length(o, 0), 1u << 17));
-        const BatchKey key = batch_key(o.seed, 0);
-        ck_tune_entry ents[32];
+    if (s.tune) {
+        // Every placement on a sample of batch 0 (nothing is committed); the
+        // fastest stays as this context's setting.
+        const uint32_t sample = static_cast<uint32_t>(std::min<uint64_t>(batch_length(s, 0), 1u << 17));
+        const BatchKey key = batch_key_of(s.seed, 0);
+        ck_tune_entry entries[32];
         uint32_t n = 0;
-        check(ck_ctx_tune(ctx.h, &p, key.data(), sample, ents, 32, &n), "ck_ctx_tune");
+        check(ck_ctx_tune(ctx.get(), &sh.shape, key.data(), sample, entries, 32, &n), "ck_ctx_tune");
         const ck_tune_entry* best = nullptr;
         for (uint32_t i = 0; i < std::min<uint32_t>(n, 32); ++i)
-            if (!best || ents[i].trials_per_s > best->trials_per_s) best = &ents[i];
-        if (best)
-            tuned = " tuned=tier" + std::to_string(best->tier) + "/layout" + std::to_string(best->layout) + "/chunk" +
-                    std::to_string(best->chunk) + "@" + std::to_string(static_cast<uint64_t>(best->trials_per_s)) +
-                    "/s";
+            if (!best || entries[i].trials_per_s > best->trials_per_s) best = &entries[i];
+        if (best) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, " tuned=%s/%s/pass%u@%.0f/s", best->tier == CK_TIER_LDS ? "lds" : "hbm",
+                          layout_name(best->layout), best->chunk, best->trials_per_s);
+            tuned = buf;
+        }
     }
-    std::unique_ptr<FILE, decltype(&fclose)> journal(open_journal(o, p, id), fclose);
-    std::vector<uint32_t> chunk(micro);
+    ck_plan plan{};
+    check(ck_ctx_plan(ctx.get(), &sh.shape, micro, &plan), "ck_ctx_plan");
+    JournalWriter journal(s, sh.shape, index);
     {
-        std::lock_guard<std::mutex> lock(print_mutex);
         hipDeviceProp_t prop{};
-        (void)hipGetDeviceProperties(&prop, device);
-        std::cerr << "Worker " << id << " GPU " << device << " (" << prop.name << ") bins=" << p.buckets
-                  << " table=" << (o.tier == "auto" ? "auto" : o.tier) << " micro=" << micro << " k=" << p.k
-                  << " stash=" << p.stash << " kick=" << p.kick << " half_bits=" << p.half_bits
-                  << " CUs=" << prop.multiProcessorCount << " lds_per_block=" << prop.sharedMemPerBlock << tuned
-                  << (o.graph ? " graph=on" : "") << "\n";
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess) prop.name[0] = '\0';
+        std::lock_guard<std::mutex> lock(sh.log);
+        // The fields the reference's harnesses read from this line
+        // (compare_cpu_gpu.py:211-215): micro=, table=, warps=, shared_bytes=,
+        // scratch_bytes=.
+        std::cerr << "Worker " << index << " GPU " << device << " (" << prop.name << ") bins=" << sh.shape.buckets
+                  << " table=" << placement_name(plan) << " micro=" << micro << " block=" << 64 * plan.chain_waves
+                  << " warps=" << plan.chain_waves << " shared_bytes=" << plan.chain_lds_bytes
+                  << " scratch_bytes=" << plan.workspace_bytes << " trials_per_pass=" << plan.trials_per_pass
+                  << " build_lds_bytes=" << plan.build_lds_bytes << " k=" << sh.shape.k
+                  << " stash=" << sh.shape.stash << " kick=" << sh.shape.kick << " half_bits=" << sh.shape.half_bits
+                  << " CUs=" << prop.multiProcessorCount << tuned << (s.graph ? " graph=on" : "") << "\n";
     }
-    while (!s.stop.load(std::memory_order_relaxed)) {
-        if (pause_requested(o)) {
-            s.stop.store(true);
+    std::vector<uint32_t> piece_failures(micro);
+    const auto launch = [&](uint32_t count, uint32_t offset, const BatchKey& key, std::vector<uint32_t>& failing) {
+        uint32_t nf = 0;
+        check(ck_launch(ctx.get(), &sh.shape, key.data(), offset, count, piece_failures.data(), &nf, nullptr),
+              "ck_launch");
+        failing.insert(failing.end(), piece_failures.begin(), piece_failures.begin() + nf);
+    };
+    const auto commit = [&](uint64_t batch, uint32_t trials, const std::vector<uint32_t>& offsets) {
+        journal.commit(batch, trials, offsets);
+    };
+    const uint64_t batches = batch_count(s);
+    for (;;) {
+        if (sh.tally.halted.load()) break;
+        if (stop_requested(s)) {
+            sh.tally.halted.store(true);
             break;
         }
-        const uint64_t bid = s.next_batch.fetch_add(1);
-        if (bid >= batches) break;
-        if (done[bid]) continue;
-        const bool committed = process_batch(
-            o, bid, micro, s,
-            [&](uint32_t n, uint32_t offset, const BatchKey& key, std::vector<uint32_t>& failures) {
-                uint32_t nf = 0;
-                check(ck_launch(ctx.h, &p, key.data(), offset, n, chunk.data(), &nf, nullptr), "ck_launch");
-                failures.insert(failures.end(), chunk.begin(), chunk.begin() + nf);
-            },
-            [&](uint64_t batch, uint32_t actual, const std::vector<uint32_t>& failures) {
-                append_record(journal.get(), batch, actual, failures);
-            });
-        if (!committed) break;
+        const uint64_t batch = sh.tally.cursor.fetch_add(1);
+        if (batch >= batches) break;
+        if (sh.committed[batch]) continue;
+        if (!process_batch(s, batch, micro, sh.tally, launch, commit)) break;
     }
 }
 
-void print_progress(const Options& o, const Counters& s, std::chrono::steady_clock::time_point start) {
-    const uint64_t initial = s.completed.load(), total = planned(o);
-    while (s.active.load() > 0) {
+// One status line a second on stderr while workers run.
+void show_progress(const Shared& sh, std::chrono::steady_clock::time_point start) {
+    const uint64_t planned = planned_trials(sh.spec), at_start = sh.tally.trials.load();
+    while (sh.tally.live_workers.load() > 0) {
         std::this_thread::sleep_for(std::chrono::seconds(1));
-        const auto done = s.completed.load(), fail = s.failed.load();
-        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-        const double rate = double(done - initial) / std::max(elapsed, 1e-6);
-        const unsigned filled = static_cast<unsigned>(std::min<uint64_t>(30, done * 30 / total));
-        std::cerr << '\r' << '[' << std::string(filled, '#') << std::string(30 - filled, '-') << "] " << std::fixed
-                  << std::setprecision(2) << 100.0 * double(done) / double(total) << "% | " << done << "/" << total
-                  << " | Fail " << fail << " | " << std::setprecision(0) << rate << " avg exp/s | elapsed "
-                  << std::setprecision(1) << elapsed << "s" << std::flush;
+        const uint64_t done = sh.tally.trials.load();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        char line[200];
+        std::snprintf(line, sizeof line, "\r%6.2f%%  %llu/%llu trials  %llu failed  %.0f trials/s  %.1f s", 100.0 * done / planned,
+                      static_cast<unsigned long long>(done), static_cast<unsigned long long>(planned),
+                      static_cast<unsigned long long>(sh.tally.failures.load()),
+                      (done - at_start) / (secs > 1e-6 ? secs : 1e-6), secs);
+        std::cerr << line << std::flush;
     }
     std::cerr << '\n';
+}
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    return n;
+}
+
+// Device memory per worker: --scratch-mib when given; else 95 % of each
+// device's free memory shared evenly by the workers placed on it (0 = the
+// library's own budget when a device holds one worker).
+std::vector<uint64_t> worker_budgets(const RunSpec& s, int gpus) {
+    std::vector<uint64_t> out(gpus, 0);
+    for (int d = 0; d < gpus; ++d) {
+        const int sharing = s.workers / gpus + (d < s.workers % gpus ? 1 : 0);
+        if (s.scratch_given) {
+            out[d] = uint64_t(s.scratch_mib) << 20;
+        } else if (sharing > 1) {
+            size_t free_b = 0, total_b = 0;
+            if (hipSetDevice(d) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+                out[d] = free_b / 20 * 19 / sharing;
+        }
+    }
+    return out;
+}
+
+std::string status_word(bool error, const LedgerState& st, const RunSpec& s) {
+    if (error) return "error";
+    if (st.trials == planned_trials(s)) return "complete";
+    if (stop_requested(s)) return "paused";
+    if (st.failures > s.stop_after) return "threshold";
+    return "ready";
+}
+
+int run(RunSpec s) {
+    const ck_params shape = derive_shape(s);
+    fs::create_directories(s.out);
+    DirectoryLock lock(s.out);
+    StopSignals signals;
+    const auto start = std::chrono::steady_clock::now();
+    LedgerState before = scan_ledger(s, shape, false);
+    std::cout << "mode=" << s.mode << " ET=" << (s.trials ? 0 : s.et) << " BT=" << s.bt << " PNUM=" << s.pnum
+              << " buckets=" << shape.buckets << " kick=" << shape.kick << " bucket_size=" << shape.bucket_size
+              << " ell_d=" << shape.ell_d << " input_mode=" << s.input_mode << " k=" << shape.k
+              << " stash=" << shape.stash << " planned=" << planned_trials(s) << "\n"
+              << "Journals: " << before.trials << " trials in " << before.batches << " batches committed, "
+              << before.failures << " failed\n";
+
+    RunReport report;
+    report.workers = s.workers;
+    {
+        ck_plan pl{};
+        const ck_placement want = placement_of(s, 0);
+        check(ck_plan_describe(&shape, &want, std::min<uint32_t>(s.micro, 1u << std::min(s.bt, 31u)), &pl),
+              "ck_plan_describe");
+        report.round_cache = pl.round_cache != 0;
+        report.placement = placement_name(pl);
+    }
+    const bool nothing_to_do = before.trials == planned_trials(s) || before.failures > s.stop_after;
+    if (s.status_only || nothing_to_do || stop_requested(s)) {
+        const LedgerState st = scan_ledger(s, shape, true);
+        report.status = status_word(false, st, s);
+        write_reports(s, shape, st, report);
+        std::cout << "Status: " << report.status << "; pending batches=" << (batch_count(s) - st.batches) << "\n";
+        return 0;
+    }
+
+    const int gpus = device_count();
+    if (gpus == 0) throw std::runtime_error("no HIP GPU is visible");
+    if (s.workers == 0) s.workers = gpus;
+    if (s.workers > gpus) std::cerr << "note: " << s.workers << " workers share " << gpus << " GPU(s)\n";
+    report.workers = s.workers;
+    // Known-answer test of the device AES on every GPU used, before any trial.
+    for (int d = 0; d < std::min(gpus, s.workers); ++d) {
+        Context ctx(d);
+        check(ck_aes_self_test(ctx.get()), "device AES-128 known-answer test");
+    }
+
+    Shared sh;
+    sh.spec = s;
+    sh.shape = shape;
+    sh.committed = std::move(before.committed);
+    sh.budgets = worker_budgets(s, gpus);
+    sh.tally.trials = before.trials;
+    sh.tally.failures = before.failures;
+    sh.tally.batches = before.batches;
+    sh.errors.resize(s.workers);
+    const int64_t work_start = steady_now_ns();
+    std::vector<std::thread> threads;
+    threads.reserve(s.workers);
+    for (int w = 0; w < s.workers; ++w) {
+        sh.tally.live_workers.fetch_add(1);
+        try {
+            threads.emplace_back([&sh, w, gpus] {
+                try {
+                    worker_main(sh, w, w % gpus);
+                } catch (...) {
+                    sh.errors[w] = std::current_exception();
+                    sh.tally.aborted.store(true);
+                    sh.tally.halted.store(true);
+                }
+                sh.tally.note_worker_exit();
+                sh.tally.live_workers.fetch_sub(1);
+            });
+        } catch (...) {
+            sh.tally.live_workers.fetch_sub(1);
+            sh.errors[w] = std::current_exception();
+            sh.tally.aborted.store(true);
+            sh.tally.halted.store(true);
+            break;
+        }
+    }
+    show_progress(sh, start);
+    for (auto& t : threads) t.join();
+
+    // The journals, not the live tallies, are the result.
+    const LedgerState st = scan_ledger(s, shape, true);
+    std::exception_ptr first_error;
+    for (const auto& e : sh.errors)
+        if (e && !first_error) first_error = e;
+    report.status = status_word(bool(first_error), st, s);
+    if (report.status == "ready") report.status = "threshold";
+    report.elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    report.work_s = std::max<int64_t>(0, sh.tally.last_exit_ns.load() - work_start) * 1e-9;
+    write_reports(s, shape, st, report);
+    std::cout << "RESULT status=" << report.status << " completed=" << st.trials << " failed=" << st.failures
+              << " elapsed_s=" << report.elapsed_s << " work_s=" << report.work_s << "\n";
+    if (report.status == "paused")
+        std::cout << "Stopped on request: batches in progress were dropped and will rerun. Run the same command "
+                     "again to resume (delete the PAUSE file first if one was used).\n";
+    if (first_error) std::rethrow_exception(first_error);
+    return 0;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
     try {
-        auto o = parse(argc, argv);
-        const auto p = parameters(o);
-        fs::create_directories(o.out);
-        RunLock run_lock(o.out);
-        PauseSignals signals;
-        Counters s;
-        auto done = resume(o, p, s);
-        const uint64_t plan = planned(o);
-        const auto start = std::chrono::steady_clock::now();
-        std::cout << "mode=" << o.mode << " ET=" << o.et << " BT=" << o.bt << " PNUM=" << o.pnum
-                  << " buckets=" << p.buckets << " kick=" << p.kick << " bucket_size=" << p.bucket_size
-                  << " ell_d=" << p.ell_d << " input_mode=" << o.input_mode << " k=" << p.k << " stash=" << p.stash
-                  << " planned=" << plan << "\n"
-                  << "Resume: " << s.completed << " completed / " << s.failed << " failed\n";
-        if (o.status_only || s.completed.load() == plan || pause_requested(o) || s.failed.load() > o.stop_after) {
-            const std::string status = s.completed.load() == plan ? "complete"
-                                       : pause_requested(o)       ? "paused"
-                                       : s.failed.load() > o.stop_after ? "threshold"
-                                                                         : "ready";
-            final_outputs(o, p, s, 0, status);
-            std::cout << "Status: " << status << "; pending batches=" << (done.size() - s.batches.load()) << "\n";
-            return 0;
-        }
-        int gpus = 0;
-        if (hipGetDeviceCount(&gpus) != hipSuccess || gpus == 0) throw std::runtime_error("No HIP GPUs");
-        if (o.workers == 0) o.workers = gpus;
-        if (o.workers < 1 || o.workers > 256) throw std::runtime_error("workers must be in 1..256");
-        if (o.workers > gpus) std::cerr << "WARNING: Multiple workers share GPUs; usually use one per GPU.\n";
-        for (int g = 0; g < std::min(gpus, o.workers); ++g) {
-            Ctx ctx(g);
-            check(ck_aes_self_test(ctx.h), "AES-128 FIPS known-answer test");
-        }
-        std::mutex print_mutex;
-        std::vector<std::thread> threads;
-        std::vector<std::exception_ptr> errors(o.workers);
-        std::exception_ptr launch_error;
-        const auto work_start = steady_ns();
-        if (s.failed.load() <= o.stop_after) {
-            for (int g = 0; g < o.workers; ++g) {
-                s.active.fetch_add(1);
-                try {
-                    threads.emplace_back([&, g]() {
-                        try {
-                            worker(g, g % gpus, o, p, done, s, print_mutex);
-                        } catch (...) {
-                            errors[g] = std::current_exception();
-                            s.abort_inflight.store(true);
-                            s.stop.store(true);
-                        }
-                        mark_finished(s);
-                        s.active.fetch_sub(1);
-                    });
-                } catch (...) {
-                    s.active.fetch_sub(1);
-                    launch_error = std::current_exception();
-                    s.abort_inflight.store(true);
-                    s.stop.store(true);
-                    break;
-                }
-            }
-            print_progress(o, s, start);
-            for (auto& t : threads) t.join();
-        }
-        Counters recovered;
-        resume(o, p, recovered);
-        const bool error =
-            bool(launch_error) || std::any_of(errors.begin(), errors.end(), [](const auto& e) { return bool(e); });
-        const std::string status = error                                 ? "error"
-                                   : recovered.completed.load() == plan ? "complete"
-                                   : pause_requested(o)                  ? "paused"
-                                                                         : "threshold";
-        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-        const double work_seconds = std::max<int64_t>(0, s.finished_ns.load() - work_start) * 1e-9;
-        final_outputs(o, p, recovered, elapsed, status, work_seconds);
-        std::cout << "RESULT status=" << status << " completed=" << recovered.completed
-                  << " failed=" << recovered.failed << " elapsed_s=" << elapsed << " work_s=" << work_seconds << "\n";
-        if (pause_requested(o))
-            std::cout << "Paused: unfinished batches were not counted. Restart the same command to resume. "
-                         "If using the PAUSE file, remove it first.\n";
-        if (launch_error) std::rethrow_exception(launch_error);
-        for (auto& e : errors)
-            if (e) std::rethrow_exception(e);
-        return 0;
+        return run(parse_command_line(argc, argv));
     } catch (const std::exception& e) {
         std::cerr << "FATAL: " << e.what() << "\n";
         return 1;

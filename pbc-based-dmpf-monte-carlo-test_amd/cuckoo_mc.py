@@ -72,6 +72,29 @@ class TuneEntry(C.Structure):
                 ("trials_per_s", C.c_double)]
 
 
+PRP_AUTO, PRP_CACHE, PRP_DIRECT = 0, 1, 2
+LDS_AUTO, LDS_MAP, LDS_ROUND = 0, 1, 2
+
+
+class Placement(C.Structure):
+    _fields_ = [("tier", C.c_uint32), ("layout", C.c_uint32), ("prp", C.c_uint32), ("lds_variant", C.c_uint32),
+                ("budget_bytes", C.c_uint64)]
+
+
+class Plan(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("tier", "layout", "round_cache", "lds_variant", "trials_per_pass",
+                                            "buffers", "windows", "chain_waves", "chain_lds_bytes",
+                                            "build_lds_bytes")] + [("workspace_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TrialInputs(C.Structure):
+    _fields_ = [("prp_key", C.c_uint32 * 44), ("walk", C.c_uint64), ("randomness", C.c_uint64),
+                ("shift", C.c_uint64), ("step", C.c_uint64)]
+
+
 class CkError(RuntimeError):
     pass
 
@@ -112,6 +135,11 @@ def _load():
         "ck_engine_failures": (C.c_int, [vp, U64, P(U64), U64, P(U64)]),
         "ck_engine_release": (C.c_int, [vp, U64]),
         "ck_engine_stats_get": (C.c_int, [vp, P(EngineStats)]),
+        "ck_trial_inputs_of": (C.c_int, [P(Params), P(U32), U32, P(TrialInputs)]),
+        "ck_round_cache": (C.c_int, [P(Params), P(U32), P(C.c_uint16)]),
+        "ck_ctx_set_placement": (C.c_int, [vp, P(Placement)]),
+        "ck_ctx_plan": (C.c_int, [vp, P(Params), U32, P(Plan)]),
+        "ck_plan_describe": (C.c_int, [P(Params), P(Placement), U32, P(Plan)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -128,7 +156,8 @@ EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "
            "ck_aes_self_test", "ck_launch",
            "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_ctx_tune", "ck_engine_create",
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
-           "ck_engine_release", "ck_engine_stats_get"]
+           "ck_engine_release", "ck_engine_stats_get", "ck_trial_inputs_of", "ck_round_cache",
+           "ck_ctx_set_placement", "ck_ctx_plan", "ck_plan_describe"]
 
 
 def _check(rc):
@@ -158,6 +187,31 @@ def host_aes(rk: np.ndarray, block) -> list:
     out = (C.c_uint32 * 4)()
     LIB.ck_host_aes(_u32p(np.ascontiguousarray(rk, dtype=np.uint32)), (C.c_uint32 * 4)(*block), out)
     return list(out)
+
+
+def trial_inputs(p: Params, rk: np.ndarray, offset: int) -> TrialInputs:
+    """Host derivation of one trial's inputs for the trial API."""
+    ti = TrialInputs()
+    _check(LIB.ck_trial_inputs_of(C.byref(p), _u32p(np.ascontiguousarray(rk, dtype=np.uint32)), offset,
+                                  C.byref(ti)))
+    return ti
+
+
+def round_cache(p: Params, prp_key) -> np.ndarray:
+    """The trial's round cache, 8 << half_bits u16 entries."""
+    key = np.ascontiguousarray(np.asarray(prp_key, dtype=np.uint32))
+    out = np.zeros(8 << p.half_bits, dtype=np.uint16)
+    _check(LIB.ck_round_cache(C.byref(p), _u32p(key), out.ctypes.data_as(C.POINTER(C.c_uint16))))
+    return out
+
+
+def plan_describe(p: Params, count: int, tier=TIER_AUTO, layout=LAYOUT_AUTO, prp=PRP_AUTO, lds_variant=LDS_AUTO,
+                  budget_bytes=0) -> Plan:
+    """The plan of a launch without a device."""
+    out = Plan()
+    want = Placement(tier, layout, prp, lds_variant, budget_bytes)
+    _check(LIB.ck_plan_describe(C.byref(p), C.byref(want), count, C.byref(out)))
+    return out
 
 
 class Context:
@@ -200,6 +254,16 @@ class Context:
 
     def set_graph(self, on: bool):
         _check(LIB.ck_ctx_set_graph(self.h, 1 if on else 0))
+
+    def set_placement(self, tier=TIER_AUTO, layout=LAYOUT_AUTO, prp=PRP_AUTO, lds_variant=LDS_AUTO,
+                      budget_bytes=0):
+        want = Placement(tier, layout, prp, lds_variant, budget_bytes)
+        _check(LIB.ck_ctx_set_placement(self.h, C.byref(want)))
+
+    def plan(self, p: Params, count: int) -> Plan:
+        out = Plan()
+        _check(LIB.ck_ctx_plan(self.h, C.byref(p), count, C.byref(out)))
+        return out
 
     def self_test(self):
         _check(LIB.ck_aes_self_test(self.h))

@@ -99,3 +99,39 @@ def test_struct_layouts():
     assert C.sizeof(ck.JobResult) == 48
     assert C.sizeof(ck.EngineConfig) == 32
     assert C.sizeof(ck.EngineStats) == 88
+
+
+def test_plan_describe_follows_the_placement_request():
+    """The planner the CLI reports from (no device): round cache or direct
+    AES as asked, the LDS tier's map or round-cache form, the budget."""
+    p = ck.params("bgh25", 16384, 16, kick=500)
+    pl = ck.plan_describe(p, 1 << 17)
+    assert (pl.tier, pl.layout, pl.round_cache) == (ck.TIER_HBM, ck.LAYOUT_DESC_PLAIN, 1)
+    assert pl.trials_per_pass == 65536 and pl.buffers == 2 and pl.chain_waves == 4 and pl.chain_lds_bytes == 0
+    assert ck.plan_describe(p, 1 << 17, prp=ck.PRP_DIRECT).round_cache == 0
+    small = ck.plan_describe(p, 1 << 17, budget_bytes=256 << 20)
+    assert small.workspace_bytes <= 256 << 20 and 0 < small.trials_per_pass < 1024
+    wide = ck.params("bgh25", 16384, 32)  # half_bits 17: no round cache fits
+    assert ck.plan_describe(wide, 1024).round_cache == 0
+    with pytest.raises(ck.CkError):
+        ck.plan_describe(wide, 1024, prp=ck.PRP_CACHE)
+    g = ck.params("drrt18", 16, 8)
+    lds = ck.plan_describe(g, 256, tier=ck.TIER_LDS, lds_variant=ck.LDS_ROUND)
+    assert (lds.tier, lds.lds_variant, lds.round_cache) == (ck.TIER_LDS, ck.LDS_ROUND, 1) and lds.chain_lds_bytes > 0
+    assert ck.plan_describe(g, 256, tier=ck.TIER_LDS).lds_variant == ck.LDS_MAP
+    with pytest.raises(ck.CkError):
+        ck.plan_describe(g, 256, tier=ck.TIER_LDS, prp=ck.PRP_DIRECT)
+
+
+def test_trial_inputs_match_the_oracle():
+    """ck_trial_inputs_of / ck_round_cache (the trial API's host inputs) equal
+    the oracle's trial setup and round cache."""
+    for mode, pnum, ell, im, seed, batch, off in [("bgh25", 16, 8, 0, 123, 7, 3), ("drrt18", 4096, 16, 1, 5, 2, 99),
+                                                  ("bgh25", 1 << 20, 20, 1, 4, 0, 12)]:
+        p = ck.params(mode, pnum, ell, input_mode=im)
+        op = ock.parameters(mode, pnum, ell, input_mode=im)
+        ti = ck.trial_inputs(p, ck.batch_key(seed, batch), off)
+        ot = ock.trial_setup(ock.batch_key(seed, batch), off, op)
+        assert list(ti.prp_key) == ot.key.words()
+        assert (ti.walk, ti.randomness, ti.shift, ti.step) == (ot.walk, ot.randomness, ot.shift, ot.step)
+        assert (ck.round_cache(p, list(ti.prp_key)) == ock.build_rounds(ot, op)).all()

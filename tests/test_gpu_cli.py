@@ -157,3 +157,52 @@ def test_cli_checkpoint_resume_n2_20(tmp_path):
     assert (whole / "failures.csv").read_text().splitlines()[1:] == (d / "failures.csv").read_text().splitlines()[1:]
     if first["status"] == "paused":
         assert 0 < first["completed"] < 1536 and first["completed"] % 512 == 0
+
+
+# The reference harness's preflight cases (compare_cpu_gpu.py:27-58): name,
+# --prp-cache, --micro, --block-threads, --table-layout, --warps-per-block,
+# --shared-variant, and the table= value the Worker line must show.
+PREFLIGHT = [
+    ("gpu_direct", "off", 257, 128, "global", 0, "auto", "direct-global-desc-plain"),
+    ("gpu_cached128", "on", 257, 128, "global", 0, "auto", "cached-global-desc-plain"),
+    ("gpu_shared_round", "on", 257, 256, "shared", 0, "round", "shared-trial"),
+    ("gpu_shared_map", "on", 257, 256, "shared", 0, "map", "shared-map"),
+    ("gpu_cached_tail", "on", 37, 256, "global", 0, "auto", "cached-global-desc-plain"),
+    ("gpu_shared_tail", "on", 5, 256, "shared", 2, "map", "shared-map"),
+]
+
+
+@pytest.mark.parametrize("mode", ["bgh25", "drrt18"])
+def test_cli_reference_harness_keys(tmp_path, fixtures, mode):
+    """Each preflight case of the reference's harness: the summary keys it
+    checks (with backend 'hip', the one key a harness run against this build
+    expects differently), the golden failures, and the Worker-line fields it
+    parses (micro=, table=, warps=, shared_bytes=, scratch_bytes=)."""
+    import re
+    want = fixtures["smoke_expected"]["failures"][mode]
+    for name, cache, micro, block, layout, warps, variant, table in PREFLIGHT:
+        d = tmp_path / name
+        args = ["--mode", mode, *SMOKE, "--input-mode", "sequential", "--workers", "1", "--micro", str(micro),
+                "--scratch-mib", "64", "--prp-cache", cache, "--block-threads", str(block), "--table-layout", layout,
+                "--shared-variant", variant, "--out", str(d)]
+        if warps:
+            args += ["--warps-per-block", str(warps)]
+        out = run(args)
+        s = summary(d)
+        expected = dict(status="complete", completed=8192, planned=8192, ET=13, BT=8, mode=mode, PNUM_T=16, ELL_I=8,
+                        seed_decimal="123", input_mode="sequential", seed_scheme="aes-batch-v1", backend="hip",
+                        prp_cache_enabled=cache == "on", table_layout_requested=layout,
+                        shared_variant_requested=variant, completed_batches=32, pending_batches=0, workers=1,
+                        scratch_mib=64)
+        for key, value in expected.items():
+            assert s.get(key) == value, (name, key, s.get(key))
+        assert s["table_layout_planned"] == table, name
+        assert s["work_elapsed_seconds"] > 0
+        assert failures(d) == want and s["failed"] == len(want), name
+        text = out.stderr
+        assert [int(x) for x in re.findall(r"Worker .*? micro=(\d+)", text)] == [micro], name
+        assert re.findall(r"table=(\S+)", text)[:1] == [table], name
+        assert len(re.findall(r"\bwarps=(\d+)", text)) == 1
+        assert int(re.findall(r"shared_bytes=(\d+)", text)[0]) > 0 if layout == "shared" else True
+        scratch = int(re.findall(r"scratch_bytes=(\d+)", text)[0])
+        assert 0 < scratch <= 64 << 20, (name, scratch)
