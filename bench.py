@@ -224,29 +224,9 @@ def prewarm(ck, ctx, pts, ps):
     ctx.launch(ps[big], ck.batch_key(pts[big]["seed"], 0), 0, min(pts[big]["trials"], 1 << 16))
 
 
-def sweep_leg(ck, points: int, trials: int, plain_too: bool = True):
-    """Mixed sweep (seed 2): one engine run over all points against one
-    single-shape launch per point, same trials; failure counts must agree."""
-    sw = load_sweep()
-    pts = sw.sweep_points(points, seed=2, trials=trials)
-    ps = [sw.params_of(ck, pt) for pt in pts]
-    plain, plain_s = None, None
-    if plain_too:
-        ctx = ck.Context(0)
-        prewarm(ck, ctx, pts, ps)
-        t0 = time.perf_counter()
-        plain = []
-        for pt, p in zip(pts, ps):
-            f, st = ctx.launch(p, ck.batch_key(pt["seed"], 0), 0, pt["trials"])
-            plain.append((len(f), st.probes))
-        plain_s = time.perf_counter() - t0
-        ctx.close()
-        print(f"sweep: per-point launches {plain_s:.2f} s", file=sys.stderr, flush=True)
-    eng = ck.Engine(0, policy=ck.SCHED_SRPT)
-    t0 = time.perf_counter()
-    ids = [eng.submit(p, pt["seed"], pt["trials"], bt=pt["bt"]) for pt, p in zip(pts, ps)]
-    # Jobs finish in the scheduler's order, not in submission order: poll all
-    # of them and report progress every 20 s.
+def wait_all(eng, ids, label, t0):
+    """Results of every job (they finish in the scheduler's order), with a
+    progress line every 20 s."""
     res = [None] * len(ids)
     left, last = len(ids), time.perf_counter()
     while left:
@@ -260,31 +240,94 @@ def sweep_leg(ck, points: int, trials: int, plain_too: bool = True):
             time.sleep(0.005)
         if time.perf_counter() - last > 20:
             last = time.perf_counter()
-            print(f"sweep: {len(ids) - left} of {len(ids)} jobs done at {last - t0:.1f} s", file=sys.stderr, flush=True)
+            print(f"{label}: {len(ids) - left} of {len(ids)} jobs done at {last - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+    return res
+
+
+def plain_loop(ck, pts, ps, trials_of):
+    """One single-shape launch per point, in order (the per-point loop of
+    tune_bench.py:104-151): wall time, per-point results, and the device time
+    the launches spanned (first kernel start to last kernel end, per launch)."""
+    ctx = ck.Context(0)
+    prewarm(ck, ctx, pts, ps)
+    t0 = time.perf_counter()
+    out, span_ms, probes = [], 0.0, 0
+    for pt, p in zip(pts, ps):
+        f, st = ctx.launch(p, ck.batch_key(pt["seed"], 0), 0, trials_of(pt))
+        out.append((len(f), st.probes))
+        span_ms += st.span_ms
+        probes += st.probes
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return out, wall, span_ms / 1e3, probes
+
+
+def sweep_run(ck, pts, trials_of, plain_too, label):
+    ps = [sw_params(ck, pt) for pt in pts]
+    plain = plain_s = busy_s = plain_probes = None
+    if plain_too:
+        plain, plain_s, busy_s, plain_probes = plain_loop(ck, pts, ps, trials_of)
+        print(f"{label}: per-point launches {plain_s:.2f} s (device busy {busy_s:.2f} s)", file=sys.stderr,
+              flush=True)
+    eng = ck.Engine(0, policy=ck.SCHED_SRPT)
+    t0 = time.perf_counter()
+    ids = [eng.submit(p, pt["seed"], trials_of(pt), bt=pt["bt"]) for pt, p in zip(pts, ps)]
+    res = wait_all(eng, ids, label, t0)
     eng_s = time.perf_counter() - t0
     st = eng.stats()
     eng.close()
-    print(f"sweep: engine {eng_s:.2f} s", file=sys.stderr, flush=True)
+    print(f"{label}: engine {eng_s:.2f} s", file=sys.stderr, flush=True)
     equal = plain is None or all((r.failed, r.probes) == pl for r, pl in zip(res, plain))
-    total = sum(pt["trials"] for pt in pts)
+    total = sum(trials_of(pt) for pt in pts)
+    probes = sum(r.probes for r in res)
     return {
-        "workload": f"{points} points from seed 2 (n log-uniform [2^8,2^16], k 2..4, m/n U[1.2,1.6], stash 0..4, "
-                    f"kick 100/500/1000, bgh25, ell_i 16), {trials} trials each",
-        "jobs_per_s": points / eng_s, "trials_per_s": total / eng_s,
-        "plain_jobs_per_s": points / plain_s if plain_s else None,
+        "jobs_per_s": len(pts) / eng_s, "trials_per_s": total / eng_s,
+        "plain_jobs_per_s": len(pts) / plain_s if plain_s else None,
         "plain_trials_per_s": total / plain_s if plain_s else None,
         "speedup_vs_plain": plain_s / eng_s if plain_s else None,
         "engine_s": eng_s, "plain_s": plain_s, "passes": st.passes,
         "device_build_ms": st.build_ms, "device_chain_ms": st.chain_ms,
-        "failed": sum(r.failed for r in res), "probes": sum(r.probes for r in res),
-        "probes_per_s": sum(r.probes for r in res) / eng_s, "results_equal_plain": equal,
+        # The per-point loop's device occupancy: the fraction of its wall time
+        # inside a launch's kernels, and its probe rate while there.
+        "plain_device_busy_s": busy_s, "plain_device_busy_frac": busy_s / plain_s if plain_s else None,
+        "plain_probes_per_busy_s": plain_probes / busy_s if busy_s else None,
+        "failed": sum(r.failed for r in res), "probes": probes,
+        "probes_per_s": probes / eng_s, "plain_probes_per_s": plain_probes / plain_s if plain_s else None,
+        "results_equal_plain": equal,
     }, equal
 
 
-def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float):
+def sw_params(ck, pt):
+    return load_sweep().params_of(ck, pt)
+
+
+SWEEP_DESC = ("points from seed 2 (n log-uniform [2^8,2^16], k 2..4, m/n U[1.2,1.6], stash 0..4, "
+              "kick 100/500/1000, bgh25, ell_i 16)")
+
+
+def sweep_leg(ck, points: int, trials: int, plain_too: bool = True):
+    """Mixed sweep (seed 2): one engine run over all points against one
+    single-shape launch per point, same trials; failure counts must agree."""
+    pts = load_sweep().sweep_points(points, seed=2, trials=trials)
+    rec, equal = sweep_run(ck, pts, lambda pt: trials, plain_too, "sweep")
+    return {"workload": f"{points} {SWEEP_DESC}, {trials} trials each", **rec}, equal
+
+
+def sweep_full_leg(ck, points: int, trials: int = 10**6):
+    """The sweep at its stated size, 10^6 trials per point, on the first
+    `points` points of the seed-2 sequence (an i.i.d. sample of the 512)."""
+    pts = load_sweep().sweep_points(points, seed=2, trials=trials)
+    rec, equal = sweep_run(ck, pts, lambda pt: trials, True, "sweep-full")
+    return {"workload": f"first {points} of the 512 {SWEEP_DESC}, {trials} trials each", "points": points,
+            **rec}, equal
+
+
+def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float, plain_jobs: int):
     """Poisson arrivals (seed 3) of sweep-shaped jobs: the engine (continuous
-    batching, least-remaining-work first) against a plain loop that launches
-    one job at a time in arrival order.  Latency = completion - arrival."""
+    batching, least-remaining-work first) over `jobs` arrivals per rate, and a
+    plain loop that launches one job at a time in arrival order over the first
+    `plain_jobs` of them.  Latency = completion - arrival."""
     sw = load_sweep()
     shapes = sw.sweep_points(jobs, seed=3)
     ps = [sw.params_of(ck, pt) for pt in shapes]
@@ -305,11 +348,12 @@ def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float):
         lat = [r.end_ms - r.submit_ms for r in res]
         span = (max(r.end_ms for r in res) - min(r.submit_ms for r in res)) / 1e3
         # Plain loop: one launch at a time, in arrival order.
+        n_plain = min(plain_jobs, jobs)
         ctx = ck.Context(0)
-        prewarm(ck, ctx, [dict(pt, trials=b) for pt, (_, b) in zip(shapes, arr)], ps)
+        prewarm(ck, ctx, [dict(pt, trials=b) for pt, (_, b) in zip(shapes[:n_plain], arr[:n_plain])], ps[:n_plain])
         t0 = time.perf_counter()
         plat, pres = [], []
-        for (at, budget), pt, p in zip(arr, shapes, ps):
+        for (at, budget), pt, p in zip(arr[:n_plain], shapes[:n_plain], ps[:n_plain]):
             d = at - (time.perf_counter() - t0)
             if d > 0:
                 time.sleep(d)
@@ -318,15 +362,67 @@ def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float):
             plat.append((time.perf_counter() - t0 - at) * 1e3)
         pspan = time.perf_counter() - t0
         ctx.close()
-        equal = all((r.failed, r.probes) == q for r, q in zip(res, pres))
+        equal = all((r.failed, r.probes) == q for r, q in zip(res[:n_plain], pres))
         ok = ok and equal
         out.append({"rate": rate, "jobs": jobs, "jobs_per_s": jobs / span, "p50_ms": pct(lat, 50),
-                    "p99_ms": pct(lat, 99), "plain_jobs_per_s": jobs / pspan, "plain_p50_ms": pct(plat, 50),
-                    "plain_p99_ms": pct(plat, 99), "results_equal_plain": equal})
-        print(f"serve: rate {rate}: p50 {pct(lat, 50):.1f} ms p99 {pct(lat, 99):.1f} ms "
-              f"(plain {pct(plat, 50):.1f} / {pct(plat, 99):.1f})", file=sys.stderr, flush=True)
+                    "p99_ms": pct(lat, 99), "plain_jobs": n_plain, "plain_jobs_per_s": n_plain / pspan,
+                    "plain_p50_ms": pct(plat, 50), "plain_p99_ms": pct(plat, 99), "results_equal_plain": equal})
+        print(f"serve: rate {rate}: {jobs} jobs, p50 {pct(lat, 50):.1f} ms p99 {pct(lat, 99):.1f} ms "
+              f"(plain, first {n_plain}: {pct(plat, 50):.1f} / {pct(plat, 99):.1f})", file=sys.stderr, flush=True)
     return {"workload": f"Poisson arrivals (seed 3), shapes as the sweep (seed 3), trial budgets log-uniform "
                         f"[{budget_lo:g}, {budget_hi:g}]", "rates": out}, ok
+
+
+def serve_stated_leg(ck, rate: float, window_s: float, budget_lo: float = 1e5, budget_hi: float = 1e9):
+    """Serving at the stated budgets (log-uniform over [1e5, 1e9], mean
+    ~1.1e8 trials per job): jobs arrive at `rate` per second (seed 3) for
+    `window_s` seconds; the engine serves them least-remaining-work first and
+    the row reports what it finished inside the window.  The offered load is
+    the submitted trials per second of the window against the trials per
+    second the engine executed: far above 1, so the queue only grows and the
+    latency of the finished jobs is that of the small ones SRPT lets through."""
+    sw = load_sweep()
+    n = int(rate * window_s * 1.5) + 16
+    shapes = sw.sweep_points(n, seed=3)
+    arr = [a for a in sw.arrivals(n, rate, seed=3, budget_lo=budget_lo, budget_hi=budget_hi) if a[0] < window_s]
+    eng = ck.Engine(0, policy=ck.SCHED_SRPT)
+    t0 = time.perf_counter()
+    ids, done = [], {}
+    for (at, budget), pt in zip(arr, shapes):
+        d = at - (time.perf_counter() - t0)
+        if d > 0:
+            time.sleep(d)
+        ids.append(eng.submit(sw.params_of(ck, pt), pt["seed"], budget))
+    while time.perf_counter() - t0 < window_s:
+        for i in ids:
+            if i not in done:
+                r = eng.poll(i)
+                if r is not None:
+                    done[i] = r
+        time.sleep(0.01)
+    for i in ids:
+        if i not in done:
+            r = eng.poll(i)
+            if r is not None:
+                done[i] = r
+    st = eng.stats()
+    elapsed = time.perf_counter() - t0
+    eng.close()
+    lat = [r.end_ms - r.submit_ms for r in done.values()]
+    submitted_trials = sum(b for _, b in arr)
+    executed = st.slots  # trial slots run in passes, finished jobs or not
+    fin_budgets = sorted(b for (_, b), i in zip(arr, ids) if i in done)
+    print(f"serve-stated: {len(done)} of {len(ids)} jobs finished in {elapsed:.1f} s", file=sys.stderr, flush=True)
+    return {
+        "workload": f"Poisson arrivals at {rate:g} jobs/s (seed 3) for {window_s:g} s, shapes as the sweep "
+                    f"(seed 3), trial budgets log-uniform [{budget_lo:g}, {budget_hi:g}]",
+        "rate": rate, "window_s": window_s, "jobs_submitted": len(ids), "jobs_finished": len(done),
+        "p50_ms_finished": pct(lat, 50), "p99_ms_finished": pct(lat, 99),
+        "largest_finished_budget": fin_budgets[-1] if fin_budgets else None,
+        "jobs_per_s": len(done) / elapsed, "executed_trials_per_s": executed / elapsed,
+        "offered_trials_per_s": submitted_trials / window_s,
+        "offered_over_capacity": (submitted_trials / window_s) / (executed / elapsed) if executed else None,
+    }
 
 
 def config1_leg(ck):
@@ -436,9 +532,16 @@ def main():
                     help="trials per sweep point (the full sweep is 10^6)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-no-plain", action="store_true", help="skip the per-point baseline of the sweep")
+    ap.add_argument("--sweep-full-points", type=int, default=16,
+                    help="points of the sweep run at 10^6 trials each, engine and per-point (0: skip)")
     ap.add_argument("--serve-rates", default="10,100,1000")
-    ap.add_argument("--serve-jobs", type=int, default=100)
-    ap.add_argument("--serve-budget", default="10,1e5", help="trial budget range lo,hi (log-uniform)")
+    ap.add_argument("--serve-jobs", type=int, default=1000, help="arrivals per rate at the small budgets")
+    ap.add_argument("--serve-plain-jobs", type=int, default=100,
+                    help="arrivals per rate for the one-at-a-time baseline loop")
+    ap.add_argument("--serve-budget", default="10,1e5", help="small trial budget range lo,hi (log-uniform)")
+    ap.add_argument("--serve-stated-window", type=float, default=30.0,
+                    help="seconds of arrivals at the stated budgets [1e5, 1e9] (0: skip)")
+    ap.add_argument("--serve-stated-rate", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--hbm-trials", type=int, default=0,
                     help="trials of the n=2^20 leg (default 0: two full memory-capped passes)")
@@ -594,10 +697,17 @@ def main():
     if not args.no_sweep:
         rec["sweep"], good = sweep_leg(ck, args.sweep_points, args.sweep_trials, not args.sweep_no_plain)
         ok = ok and good
+        if args.sweep_full_points:
+            rec["sweep_full"], good = sweep_full_leg(ck, args.sweep_full_points)
+            ok = ok and good
+            rec["config"]["sweep_full_points"] = args.sweep_full_points
     if not args.no_serve:
         lo, hi = (float(x) for x in args.serve_budget.split(","))
-        rec["serving"], good = serve_leg(ck, [float(r) for r in args.serve_rates.split(",")], args.serve_jobs, lo, hi)
+        rec["serving"], good = serve_leg(ck, [float(r) for r in args.serve_rates.split(",")], args.serve_jobs, lo, hi,
+                                         args.serve_plain_jobs)
         ok = ok and good
+        if args.serve_stated_window > 0:
+            rec["serving"]["stated_budgets"] = serve_stated_leg(ck, args.serve_stated_rate, args.serve_stated_window)
     if not args.no_graph:
         # Last: a graph-mode context leaves its workspace allocated until the
         # process ends, which would shrink the passes of any later leg.

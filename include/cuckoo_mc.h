@@ -75,6 +75,8 @@ typedef struct ck_stats {
     double fused_ms;      /* device time of fused (LDS tier) kernels */
     uint32_t launches;    /* kernel launches */
     uint32_t graph;       /* 0: launched kernel by kernel; 1: replayed this context's graph; 2: built and ran a new one */
+    double span_ms;       /* device time from the start of the call's first kernel to the end of its last
+                             (overlapped builds and chains counted once) */
 } ck_stats;
 
 typedef struct ck_ctx ck_ctx;

@@ -375,7 +375,6 @@ struct Workspace {
     void* base = nullptr;
     size_t bytes = 0;
 };
-
 // Where a kernel goes: onto a stream, into a graph as a new node after `deps`,
 // or into an existing node of an instantiated graph (new argument values; the
 // kernel and its launch shape stay).  All three pass the arguments as one
@@ -1146,6 +1145,7 @@ static int run_hbm_graph(Ctx& c, const ck_params& p, uint32_t first, uint32_t co
         float ms = 0;
         CKH(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
         st->chain_ms += ms;
+        st->span_ms += ms;
         st->launches += 3 * passes;
         st->graph = same ? 1 : 2;
     }
@@ -1208,6 +1208,10 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
                    ck_stats* st) {
     c.budget = hbm_budget(c);
+    // A workspace within 3 % of the budget is planned into as it is: growing it
+    // would free and map hundreds of GiB again (seconds at the memory-capped
+    // n = 2^20 shape) for a few more chains per pass.
+    if (!c.budget_fixed && c.ws.bytes < c.budget && c.ws.bytes >= c.budget / 100 * 97) c.budget = c.ws.bytes;
     const bool graph_wanted =
         c.graph_on && !out.probes && !out.tables && !out.keys && !out.walks && !out.rnds && !out.map && !out.digests;
     HbmDecision d;
@@ -1263,6 +1267,19 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     }
     if (graph)
         return run_hbm_graph(c, p, first, count, layout, nbuf, chunk, h, bp, args, taggable, tag_bits, out, st);
+    // The call's device span: from before the first build to the end of the
+    // last chain (the chain-end event recorded last).
+    CKH(hipEventRecord(c.ev[2], c.build_stream));
+    hipEvent_t last_end = nullptr;
+    auto span = [&]() -> int {
+        if (st && last_end) {
+            float ms = 0;
+            CKH(hipEventSynchronize(last_end));
+            CKH(hipEventElapsedTime(&ms, c.ev[2], last_end));
+            st->span_ms += ms;
+        }
+        return CK_OK;
+    };
     auto enqueue = [&](uint32_t i) -> int {
         const int b = nbuf == 2 ? int(i & 1) : 0;
         PassArgs& a = args[b];
@@ -1285,6 +1302,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         CKH(cd.err);
         CKH(hipGetLastError());
         CKH(hipEventRecord(c.ev_chain[b][1], c.stream));
+        last_end = c.ev_chain[b][1];
         if (a.digests) {
             switch (p.k) {
                 case 2: launch_digest_k<2>(h, a, cd); break;
@@ -1385,6 +1403,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
             CKH(hipGetLastError());
             CKH(hipEventRecord(c.ev_chain[e][1], c.stream));
         }
+        last_end = c.ev_chain[(h.nwin - 1) & 1][1];
         for (uint32_t j = h.nwin - std::min<uint32_t>(h.nwin, 2); j < h.nwin; ++j)
             if ((rc = account(j))) return rc;
         if (a.digests) {
@@ -1403,19 +1422,20 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     if (h.nwin > 1) {
         for (uint32_t i = 0; i < passes; ++i)
             if ((rc = windowed(i))) return rc;
-        return CK_OK;
+        return span();
     }
     if (nbuf == 1) {
         for (uint32_t i = 0; i < passes; ++i)
             if ((rc = enqueue(i)) || (rc = gather(i, true))) return rc;
-        return CK_OK;
+        return span();
     }
     if ((rc = enqueue(0))) return rc;
     for (uint32_t i = 1; i < passes; ++i) {
         if ((rc = enqueue(i))) return rc;
         if ((rc = gather(i - 1, true))) return rc;
     }
-    return gather(passes - 1, true);
+    if ((rc = gather(passes - 1, true))) return rc;
+    return span();
 }
 
 // LDS tier: one fused persistent launch per chunk.
@@ -1472,6 +1492,7 @@ static int run_lds(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
             float f = 0;
             CKH(hipEventElapsedTime(&f, c.ev[0], c.ev[1]));
             st->fused_ms += f;
+            st->span_ms += f;
             st->launches += 1;
         }
         done += n;

@@ -37,7 +37,7 @@ class Stats(C.Structure):
     _fields_ = [("trials", C.c_uint64), ("failed", C.c_uint64), ("probes", C.c_uint64),
                 ("tier", C.c_uint32), ("chunk", C.c_uint32), ("build_ms", C.c_double),
                 ("chain_ms", C.c_double), ("fused_ms", C.c_double), ("launches", C.c_uint32),
-                ("graph", C.c_uint32)]
+                ("graph", C.c_uint32), ("span_ms", C.c_double)]
 
 
 class Job(C.Structure):

@@ -103,3 +103,31 @@ def test_windows_leave_other_layouts_and_outputs_alone(ctx):
     finally:
         ctx.set_window(0)
         ctx.set_layout(ck.LAYOUT_RESIDUAL)
+
+
+# n = 2^20, m = ceil(1.3 n), seed 4 with kick bounds low enough that trials
+# fail: (kick 75, stash 0) fails 12 of the first 32 trials, (kick 60, stash 3)
+# fails 25 of them with the stash in use (2.9 stashed items per trial).
+N2_20_FAILING = [dict(buckets=1363149, kick=75, stash=0), dict(buckets=1363149, kick=60, stash=3)]
+
+
+@pytest.mark.parametrize("kw", N2_20_FAILING, ids=["kick75", "kick60-stash3"])
+def test_n2_20_failure_detection(kw):
+    """Failure detection at the rare-event size, against the oracle trial by
+    trial (failing offsets and per-trial probes): the default launch, item
+    windows forced small, and an engine job."""
+    trials = 32
+    p = ck.params("bgh25", 1 << 20, 20, **kw)
+    of, oprobes, _ = ock.run_range(ock.batch_key(4, 0), 0, trials, ock.parameters("bgh25", 1 << 20, 20, **kw),
+                                   probes=True, threads=8)
+    assert 0 < len(of) < trials
+    with ck.Context(0) as c:
+        f, pr, _, _ = c.launch_debug(p, ck.batch_key(4, 0), 0, trials)
+        assert list(f) == list(of) and (pr == oprobes).all()
+        c.set_window(100000)  # 11 windows per trial
+        f, st = c.launch(p, ck.batch_key(4, 0), 0, trials)
+        assert list(f) == list(of) and st.probes == int(oprobes.sum())
+    with ck.Engine(0, arena_bytes=16 << 30, max_slots=1024) as eng:
+        jid = eng.submit(p, 4, trials, keep_failures=True)
+        r = eng.wait(jid)
+        assert list(map(int, eng.failures(jid))) == list(map(int, of)) and r.probes == int(oprobes.sum())

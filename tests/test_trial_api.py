@@ -81,7 +81,7 @@ def test_trial_api_in_a_kernel(fixtures):
     fn = lib.trial_api_device_run
     P = C.POINTER
     fn.restype = C.c_int
-    fn.argtypes = [C.c_uint32] * 6 + [C.c_uint64] * 4 + [C.c_uint32, P(C.c_uint64), P(C.c_uint16), P(C.c_uint32),
+    fn.argtypes = [C.c_uint32] * 7 + [C.c_uint64] * 4 + [C.c_uint32, P(C.c_uint64), P(C.c_uint16), P(C.c_uint32),
                                                          P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     groups = defaultdict(list)
     for r in fixture_rows(fixtures):
