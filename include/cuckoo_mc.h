@@ -218,6 +218,23 @@ int ck_aes_self_test(ck_ctx* ctx);
 int ck_launch(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t first,
               uint32_t count, uint32_t* failed_offsets, uint32_t* n_failed, ck_stats* stats);
 
+/* Allocation-free form of ck_launch.  It has no counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops or harnesses that forbid hipMalloc).  The launch's
+ * workspace is a CK_STATIC_SCRATCH_BYTES device array that the library holds on
+ * every GPU it runs on, from load until the process ends, instead of memory it
+ * allocates: the call makes no hipMalloc, hipFree or other allocating call.
+ * Results equal ck_launch's.  Passes are sized to the array (a few hundred
+ * trials per pass at n = 2^14, so slower than ck_launch), graph mode is not
+ * used and tables are cleared every pass.  CK_EINVAL, before any device work,
+ * when one trial of the shape needs more workspace than the array holds.  All
+ * calls on one GPU share its array and leave their contents in it: calls that
+ * overlap on one GPU (other threads or contexts) race, so the caller must
+ * serialise them. */
+#define CK_STATIC_SCRATCH_BYTES (256ull << 20)
+int ck_launch_static(ck_ctx* ctx, const ck_params* p, const uint32_t round_keys[44], uint32_t first,
+                     uint32_t count, uint32_t* failed_offsets, uint32_t* n_failed, ck_stats* stats);
+
 /* Test hook: like ck_launch, but also returns per-trial probe counts
  * (probes[i] for trial first+i) and, when tables is non-NULL and the shape
  * allows, the final table of every trial (tables[i*buckets + b], entry =

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <tuple>
@@ -375,6 +376,32 @@ struct Workspace {
     void* base = nullptr;
     size_t bytes = 0;
 };
+
+// The workspace of ck_launch_static: a file-scope device array, so that
+// launch allocates nothing.  Every GPU the library runs on holds its own copy
+// from load until the process ends.  Host code never uses the name `scratch`
+// as an address (on the host it only names the symbol); static_scratch_of()
+// resolves this device's address once, with the device current, and kernels
+// receive that pointer.
+static __device__ __align__(256) unsigned char scratch[CK_STATIC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices] = {};
+
+static int static_scratch_of(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(CK_EINVAL, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* addr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !addr)
+            return fail(CK_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[device] = addr;
+    }
+    *out = g_scratch_addr[device];
+    return CK_OK;
+}
+
 // Where a kernel goes: onto a stream, into a graph as a new node after `deps`,
 // or into an existing node of an instantiated graph (new argument values; the
 // kernel and its launch shape stay).  All three pass the arguments as one
@@ -457,6 +484,8 @@ struct Ctx {
     uint32_t prp = CK_PRP_AUTO;          // round cache or direct AES for bucket evaluations
     uint32_t lds_variant = CK_LDS_AUTO;  // LDS tier: bucket map or round cache per probe
     size_t budget_fixed = 0;             // workspace budget set by the caller (0: from the free memory)
+    void* static_scratch = nullptr;      // this device's address of `scratch` (ck_launch_static)
+    bool ws_static = false;              // the workspace is `scratch`: never grown, freed or reused across calls
     size_t budget = size_t(96) << 30;  // HBM budget for the double-buffered maps and tables (set per call)
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
@@ -732,6 +761,10 @@ static void forget_tables(Ctx& c) {
 // one).  CK_ENOMEM when the device cannot hold `bytes`.
 static int ensure_ws(Ctx& c, size_t bytes) {
     if (c.ws.bytes >= bytes) return CK_OK;
+    if (c.ws_static)
+        return fail(CK_EINVAL, "this launch needs a " + std::to_string(bytes >> 20) +
+                                   " MiB workspace; ck_launch_static's array holds " +
+                                   std::to_string(c.ws.bytes >> 20) + " MiB (one trial of this shape does not fit)");
     forget_tables(c);
     for (HbmGraph& g : c.graphs)
         if (g.exec && g.ws == c.ws.base) g.drop();
@@ -1258,7 +1291,8 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         st->chunk = chunk;
     }
     const uint32_t passes = (count + chunk - 1) / chunk;
-    const bool taggable = layout == CK_LAYOUT_DESC_PLAIN && !out.digests;
+    // Tables are reused across passes only in a workspace this context owns.
+    const bool taggable = layout == CK_LAYOUT_DESC_PLAIN && !out.digests && !c.ws_static;
     const uint32_t tag_bits = taggable ? 62 - p.k * args[0].S.dbits : 0;
     const uint64_t plan = table_plan(c, p, h, nbuf, taggable);
     if (plan != c.table_plan) {
@@ -1592,7 +1626,8 @@ int ck_ctx_create(int device, ck_ctx** out) {
         ctx->c.lds_cu = std::max(ctx->c.lds_block, static_cast<size_t>(v));
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0)
         ctx->c.cus = static_cast<uint32_t>(v);
-    const int rc = set_lds_limits(ctx->c.lds_block);
+    int rc = set_lds_limits(ctx->c.lds_block);
+    if (rc == CK_OK) rc = static_scratch_of(device, &ctx->c.static_scratch);
     if (rc) {
         ck_ctx_destroy(ctx);
         return rc;
@@ -1702,6 +1737,28 @@ static int launch_common(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44],
 int ck_launch(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t first, uint32_t count,
               uint32_t* failed_offsets, uint32_t* n_failed, ck_stats* stats) {
     return launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, nullptr, nullptr, nullptr, stats);
+}
+
+int ck_launch_static(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t first, uint32_t count,
+                     uint32_t* failed_offsets, uint32_t* n_failed, ck_stats* stats) {
+    if (!ctx || !ctx->c.static_scratch) return fail(CK_EINVAL, "null ctx");
+    Ctx& c = ctx->c;
+    // The context's own workspace and settings are set aside for the call.
+    const Workspace own = c.ws;
+    const size_t own_budget = c.budget_fixed;
+    const bool own_graph = c.graph_on;
+    forget_tables(c);
+    c.ws = Workspace{c.static_scratch, CK_STATIC_SCRATCH_BYTES};
+    c.ws_static = true;
+    c.budget_fixed = CK_STATIC_SCRATCH_BYTES;
+    c.graph_on = false;
+    const int rc = launch_common(ctx, p, rk, first, count, failed_offsets, n_failed, nullptr, nullptr, nullptr, stats);
+    forget_tables(c);
+    c.ws = own;
+    c.ws_static = false;
+    c.budget_fixed = own_budget;
+    c.graph_on = own_graph;
+    return rc;
 }
 
 int ck_launch_debug(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t first, uint32_t count,

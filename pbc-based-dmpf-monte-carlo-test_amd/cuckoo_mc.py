@@ -122,6 +122,7 @@ def _load():
                                        P(Stats)]),
         "ck_aes_self_test": (C.c_int, [vp]),
         "ck_launch": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(Stats)]),
+        "ck_launch_static": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(Stats)]),
         "ck_launch_debug": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U32),
                                       P(Stats)]),
         "ck_trial_map": (C.c_int, [vp, P(Params), P(U32), U32, P(U32), P(U64), P(U64), P(U32)]),
@@ -157,7 +158,7 @@ EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "
            "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_ctx_tune", "ck_engine_create",
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
            "ck_engine_release", "ck_engine_stats_get", "ck_trial_inputs_of", "ck_round_cache",
-           "ck_ctx_set_placement", "ck_ctx_plan", "ck_plan_describe"]
+           "ck_ctx_set_placement", "ck_ctx_plan", "ck_plan_describe", "ck_launch_static"]
 
 
 def _check(rc):
@@ -285,6 +286,16 @@ class Context:
         n = C.c_uint32()
         st = Stats()
         _check(LIB.ck_launch(self.h, C.byref(p), _u32p(rk), first, count, _u32p(out), C.byref(n), C.byref(st)))
+        return out[:n.value].copy(), st
+
+    def launch_static(self, p: Params, rk: np.ndarray, first: int, count: int):
+        """ck_launch on the library's static device array (no allocation)."""
+        rk = np.ascontiguousarray(rk, dtype=np.uint32)
+        out = np.zeros(max(count, 1), dtype=np.uint32)
+        n = C.c_uint32()
+        st = Stats()
+        _check(LIB.ck_launch_static(self.h, C.byref(p), _u32p(rk), first, count, _u32p(out), C.byref(n),
+                                    C.byref(st)))
         return out[:n.value].copy(), st
 
     def launch_debug(self, p: Params, rk: np.ndarray, first: int, count: int, tables=False):
