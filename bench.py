@@ -52,8 +52,14 @@ OWN_ITEM_BYTES = 8
 # probes (the default at this shape), from FETCH_SIZE and WRITE_SIZE per
 # dispatch divided by the dispatch's probes; the chain's reads are 64-byte
 # requests, which FETCH_SIZE counts exactly (profiles/k_chain_hbm_n16384_desc_traffic.md).
-TRAFFIC_READ_PER_PROBE = 64.45
+TRAFFIC_READ_PER_PROBE = 64.40
 TRAFFIC_WRITE_PER_PROBE = 27.48
+# DRAM requests (TCC_EA0_RDREQ + TCC_EA0_WRREQ) per unit, from rocprofv3 --pmc:
+# k_chain per probe (profiles/k_chain_hbm_n16384_desc_traffic.md), ck_randbench
+# per read + write step and per read-only step (profiles/randbench_192k_traffic.md).
+CHAIN_REQ_PER_PROBE = 1.0086 + 0.8584
+RB_READ_WRITE_REQ_PER_STEP = 0.998 + 1.0005
+RB_READ_ONLY_REQ_PER_STEP = 0.998
 
 
 def int_ops_per_trial(p, probes_per_trial: float) -> float:
@@ -192,12 +198,24 @@ def random_access_ceiling(probes_per_s: float):
         sweep.setdefault(int(lanes), {})[form] = float(rate)
     best_lanes = max(sweep, key=lambda n: sweep[n].get("read_write", 0))
     rw = sweep[best_lanes]["read_write"]
-    return {"pattern": "lanes x 192 KB tables, dependent random 8-byte read + write per step "
+    ro = max(v.get("read_only", 0) for v in sweep.values())
+    chain_req = probes_per_s * CHAIN_REQ_PER_PROBE
+    ceiling_req = ro * RB_READ_ONLY_REQ_PER_STEP
+    return {"pattern": "lanes x 192 KB tables, dependent random 8-byte read (+ write) per step "
                        "(load -> xor -> mul_hi -> address)",
-            "read_write_steps_per_s": rw, "best_lanes": best_lanes,
-            "read_only_steps_per_s": max(v.get("read_only", 0) for v in sweep.values()),
+            "read_write_steps_per_s": rw, "best_lanes": best_lanes, "read_only_steps_per_s": ro,
             "sweep": [{"lanes": n, **sweep[n]} for n in sorted(sweep)],
-            "chain_probes_per_s": probes_per_s, "frac": probes_per_s / rw}
+            "chain_probes_per_s": probes_per_s,
+            "chain_over_read_write_steps": probes_per_s / rw,
+            # The bound is the DRAM random-request rate: the chain's requests per
+            # second against what random reads alone get (the most the memory
+            # system gave any random pattern measured), and against read + write.
+            "chain_dram_requests_per_s": chain_req,
+            "read_only_requests_per_s": ceiling_req,
+            "read_write_requests_per_s": rw * RB_READ_WRITE_REQ_PER_STEP,
+            "frac": chain_req / ceiling_req,
+            "note": "requests per probe / per step are profile-derived (TCC_EA0_RDREQ + WRREQ): chain 1.867, "
+                    "read + write step 1.999, read-only step 0.998"}
 
 
 def load_sweep():

@@ -140,10 +140,11 @@ int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
  * Results are those of the kernel-by-kernel form.  Launches that ask for
  * per-trial outputs (ck_launch_debug, ck_launch_digest) stay kernel by kernel.
  * With graph mode, stats->chain_ms is the device time of the whole replay and
- * stats->build_ms is 0.  On ROCm 7.2 the HIP runtime keeps the workspace of a
- * context that has replayed graphs allocated after ck_ctx_destroy (hipFree
- * returns success but the memory stays in use until the process ends): use
- * graph mode in long-lived contexts. */
+ * stats->build_ms is 0.  On ROCm 7.2 the HIP runtime keeps an allocation that
+ * an instantiated graph's kernel nodes pointed into after hipFree, until the
+ * process ends; the library therefore keeps such workspaces itself and hands
+ * them to the next context on the device, so a process holds one of them per
+ * device, not one per graph-mode context. */
 int ck_ctx_set_graph(ck_ctx* ctx, uint32_t on);
 
 /* How bucket evaluations get their Feistel round function. */

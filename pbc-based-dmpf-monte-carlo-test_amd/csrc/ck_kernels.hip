@@ -402,6 +402,39 @@ static int static_scratch_of(int device, void** out) {
     return CK_OK;
 }
 
+// Workspaces that an instantiated graph's kernel nodes have pointed into.  On
+// ROCm 7.2 the runtime keeps such an allocation after hipFree, even once the
+// graph instances are destroyed, the streams synchronised and destroyed and
+// the graph and default memory pools trimmed (measured: each graph-mode
+// context left its 40 GiB workspace behind, cumulatively).  The library
+// therefore never frees them: it keeps them per device and hands them to the
+// next context on that device that needs a workspace they can hold.
+static std::mutex g_retired_mu;
+static std::vector<Workspace> g_retired[kMaxDevices];
+
+static void retire_ws(int device, const Workspace& w) {
+    std::lock_guard<std::mutex> lock(g_retired_mu);
+    g_retired[device].push_back(w);
+}
+// The smallest retired workspace of at least `bytes`, removed from the pool.
+static bool take_retired(int device, size_t bytes, Workspace& out) {
+    std::lock_guard<std::mutex> lock(g_retired_mu);
+    auto& pool = g_retired[device];
+    auto best = pool.end();
+    for (auto it = pool.begin(); it != pool.end(); ++it)
+        if (it->bytes >= bytes && (best == pool.end() || it->bytes < best->bytes)) best = it;
+    if (best == pool.end()) return false;
+    out = *best;
+    pool.erase(best);
+    return true;
+}
+static size_t largest_retired(int device) {
+    std::lock_guard<std::mutex> lock(g_retired_mu);
+    size_t m = 0;
+    for (const auto& w : g_retired[device]) m = std::max(m, w.bytes);
+    return m;
+}
+
 // Where a kernel goes: onto a stream, into a graph as a new node after `deps`,
 // or into an existing node of an instantiated graph (new argument values; the
 // kernel and its launch shape stay).  All three pass the arguments as one
@@ -486,6 +519,7 @@ struct Ctx {
     size_t budget_fixed = 0;             // workspace budget set by the caller (0: from the free memory)
     void* static_scratch = nullptr;      // this device's address of `scratch` (ck_launch_static)
     bool ws_static = false;              // the workspace is `scratch`: never grown, freed or reused across calls
+    bool ws_graphed = false;             // an instantiated graph has pointed into the workspace: retire, never free
     size_t budget = size_t(96) << 30;  // HBM budget for the double-buffered maps and tables (set per call)
     size_t lds_block = 64 * 1024;      // dynamic LDS per block the device allows
     size_t lds_cu = 64 * 1024;         // LDS per CU
@@ -722,7 +756,8 @@ static size_t hbm_budget(const Ctx& c) {
     if (c.budget_fixed) return c.budget_fixed;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return size_t(96) << 30;
-    return std::min(kBudgetCap, (free_b + c.ws.bytes) / 20 * 19);
+    const size_t own = std::max(c.ws.bytes, largest_retired(c.device));  // memory a growing workspace can use
+    return std::min(kBudgetCap, (free_b + own) / 20 * 19);
 }
 
 // LDS-tier per-launch scratch: [counters | probes C | fail C | digests C | tables C x buckets].
@@ -768,9 +803,20 @@ static int ensure_ws(Ctx& c, size_t bytes) {
     forget_tables(c);
     for (HbmGraph& g : c.graphs)
         if (g.exec && g.ws == c.ws.base) g.drop();
-    if (c.ws.base) CKH(hipFree(c.ws.base));
-    c.ws.base = nullptr;
-    c.ws.bytes = 0;
+    if (c.ws.base) {
+        if (c.ws_graphed)
+            retire_ws(c.device, c.ws);
+        else
+            CKH(hipFree(c.ws.base));
+    }
+    c.ws = Workspace{};
+    c.ws_graphed = false;
+    Workspace reuse;
+    if (take_retired(c.device, bytes, reuse)) {
+        c.ws = reuse;
+        c.ws_graphed = true;
+        return CK_OK;
+    }
     const hipError_t e = hipMalloc(&c.ws.base, bytes);
     if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
@@ -1136,6 +1182,7 @@ static int run_hbm_graph(Ctx& c, const ck_params& p, uint32_t first, uint32_t co
             G.chain[i] = cd.node;
         }
         CKH(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
+        c.ws_graphed = true;
         G.p = p;
         G.ws = c.ws.base;
         G.count = count;
@@ -1643,7 +1690,12 @@ void ck_ctx_destroy(ck_ctx* ctx) {
         g.drop();
         if (g.aux) (void)hipFree(g.aux);
     }
-    if (ctx->c.ws.base) (void)hipFree(ctx->c.ws.base);
+    if (ctx->c.ws.base) {
+        if (ctx->c.ws_graphed)
+            retire_ws(ctx->c.device, ctx->c.ws);
+        else
+            (void)hipFree(ctx->c.ws.base);
+    }
     for (auto& ev : ctx->c.ev)
         if (ev) (void)hipEventDestroy(ev);
     for (int b = 0; b < 2; ++b)

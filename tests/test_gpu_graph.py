@@ -96,3 +96,32 @@ def test_graph_full_batch_matches_at_headline_shape():
             assert list(map(int, f)) == list(map(int, ef))
             assert st.probes == est.probes
 
+
+
+def device_free_bytes():
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_graph_contexts_do_not_accumulate_memory():
+    """The runtime keeps a workspace that graph nodes pointed into after it is
+    freed; the library keeps such workspaces for reuse instead, so a sequence
+    of graph-mode contexts (and an eager one after them) holds one workspace,
+    not one per context.  Results stay those of kernel-by-kernel launches."""
+    p = ck.params("bgh25", 4096, 16, kick=500)
+    with ck.Context(0) as ctx:
+        want, _ = ctx.launch(p, ck.batch_key(3, 0), 0, 1 << 16)
+    free_after = []
+    for graph in (True, True, True, False):
+        with ck.Context(0) as ctx:
+            ctx.set_graph(graph)
+            for _ in range(2):
+                f, st = ctx.launch(p, ck.batch_key(3, 0), 0, 1 << 16)
+                assert list(f) == list(want)
+            assert st.graph == (1 if graph else 0)
+        free_after.append(device_free_bytes())
+    slack = 1 << 30
+    assert all(f >= free_after[0] - slack for f in free_after[1:]), [f >> 30 for f in free_after]
