@@ -43,21 +43,21 @@ PROBE_OPS = 20         # choice draw (div/mod by k), table read, compare, write,
 # build, or skipped through pass tags, so not part of k_chain's launch).
 ALGO_PROBE_BYTES = 8
 ALGO_BUCKET_INIT_BYTES = 4
-# The bytes this build's own descriptor layout moves per probe and per item:
-# an 8-byte slot read and written back, and each new item's 8-byte descriptor
-# read once from the insertion-order stream (reported beside the roofline).
-OWN_PROBE_BYTES = 16
-OWN_ITEM_BYTES = 8
-# HBM bytes per probe of k_chain in the descriptor layout with load + store
-# probes (the default at this shape), from FETCH_SIZE and WRITE_SIZE per
-# dispatch divided by the dispatch's probes; the chain's reads are 64-byte
-# requests, which FETCH_SIZE counts exactly (profiles/k_chain_hbm_n16384_desc_traffic.md).
-TRAFFIC_READ_PER_PROBE = 64.40
-TRAFFIC_WRITE_PER_PROBE = 27.48
-# DRAM requests (TCC_EA0_RDREQ + TCC_EA0_WRREQ) per unit, from rocprofv3 --pmc:
-# k_chain per probe (profiles/k_chain_hbm_n16384_desc_traffic.md), ck_randbench
-# per read + write step and per read-only step (profiles/randbench_192k_traffic.md).
-CHAIN_REQ_PER_PROBE = 1.0086 + 0.8584
+# Per table layout at the headline shape: the chain kernel, the bytes this
+# build's own layout moves per probe (its slot read and written back) and per
+# inserted item (the item's candidate word, read once from the insertion-order
+# stream), and the profile-derived HBM bytes and DRAM requests
+# (TCC_EA0_RDREQ + TCC_EA0_WRREQ) per probe from rocprofv3 --pmc.  The chain's
+# reads are 64-byte requests, which FETCH_SIZE counts exactly.
+CHAIN_PROFILES = {
+    "desc-plain": dict(kernel="k_chain<3, bgh, u64 descriptor stream, u64 entries>", slot_bytes=16, item_bytes=8,
+                       read=64.40, write=27.48, requests=1.0086 + 0.8584,
+                       profile="profiles/k_chain_hbm_n16384_desc_traffic.md"),
+    "residual": dict(kernel="k_chain_res<3, bgh, u64 candidate stream, u32 entries>", slot_bytes=8, item_bytes=8,
+                     read=65.22, write=27.45, requests=1.0185 + 0.8571,
+                     profile="profiles/k_chain_res_hbm_n16384_u32_traffic.md"),
+}
+# DRAM requests per step of ck_randbench (profiles/randbench_192k_traffic.md).
 RB_READ_WRITE_REQ_PER_STEP = 0.998 + 1.0005
 RB_READ_ONLY_REQ_PER_STEP = 0.998
 
@@ -176,7 +176,7 @@ def cpu_reference(cores: int, gpu_failures_prefix):
             "gpu_failures_equal": failed == gpu_failures_prefix}
 
 
-def random_access_ceiling(probes_per_s: float):
+def random_access_ceiling(probes_per_s: float, requests_per_probe):
     """Runs lib/ck_randbench: the chain's access pattern (each lane a dependent
     random read + write of one 8-byte slot of its own 192 KB table, with one
     XOR and one 32-bit multiply-high between a load and the next address)
@@ -199,7 +199,7 @@ def random_access_ceiling(probes_per_s: float):
     best_lanes = max(sweep, key=lambda n: sweep[n].get("read_write", 0))
     rw = sweep[best_lanes]["read_write"]
     ro = max(v.get("read_only", 0) for v in sweep.values())
-    chain_req = probes_per_s * CHAIN_REQ_PER_PROBE
+    chain_req = probes_per_s * requests_per_probe if requests_per_probe else None
     ceiling_req = ro * RB_READ_ONLY_REQ_PER_STEP
     return {"pattern": "lanes x 192 KB tables, dependent random 8-byte read (+ write) per step "
                        "(load -> xor -> mul_hi -> address)",
@@ -213,9 +213,9 @@ def random_access_ceiling(probes_per_s: float):
             "chain_dram_requests_per_s": chain_req,
             "read_only_requests_per_s": ceiling_req,
             "read_write_requests_per_s": rw * RB_READ_WRITE_REQ_PER_STEP,
-            "frac": chain_req / ceiling_req,
-            "note": "requests per probe / per step are profile-derived (TCC_EA0_RDREQ + WRREQ): chain 1.867, "
-                    "read + write step 1.999, read-only step 0.998"}
+            "frac": chain_req / ceiling_req if chain_req else None,
+            "note": f"requests per probe / per step are profile-derived (TCC_EA0_RDREQ + WRREQ): chain "
+                    f"{requests_per_probe}, read + write step 1.999, read-only step 0.998"}
 
 
 def load_sweep():
@@ -607,6 +607,9 @@ def main():
         del src, dst
         print(f"copy probe: {args.copy_probe_mib} MiB read + written", file=sys.stderr, flush=True)
 
+    # The table layout the library plans for a step (auto picks by shape).
+    layout_name = {ck.LAYOUT_MAP: "map", ck.LAYOUT_DESC: "desc", ck.LAYOUT_MAP_XCHG: "map-xchg",
+                   ck.LAYOUT_DESC_PLAIN: "desc-plain", ck.LAYOUT_RESIDUAL: "residual"}.get(ctx.plan(p, t).layout, "?")
     keys = {}
 
     def step(i):
@@ -641,10 +644,10 @@ def main():
     probes_per_launch = agg["probes"] / chain_launches
     bytes_per_launch = ALGO_PROBE_BYTES * probes_per_launch
     achieved = bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else 0.0
-    own_bytes_per_launch = (OWN_PROBE_BYTES * agg["probes"] + OWN_ITEM_BYTES * p.pnum * trials) / chain_launches
-    measured_layout = args.layout in ("auto", "desc-plain")
-    traffic = ((TRAFFIC_READ_PER_PROBE + TRAFFIC_WRITE_PER_PROBE) * probes_per_launch
-               if measured_layout else None)
+    prof = CHAIN_PROFILES.get(layout_name)
+    own_bytes_per_launch = ((prof["slot_bytes"] * agg["probes"] + prof["item_bytes"] * p.pnum * trials) / chain_launches
+                            if prof else None)
+    traffic = ((prof["read"] + prof["write"]) * probes_per_launch if prof and prof["read"] is not None else None)
     probes_per_trial = agg["probes"] / trials
     ops = int_ops_per_trial(p, probes_per_trial)
     rec = {
@@ -662,14 +665,14 @@ def main():
         "data": "synthetic: per-trial keys AES-derived on device from (seed 1, batch, offset)",
         "config": {"workload": "bgh25 n=16384 m=24576 k=3 stash=0 kick=500 ell_i=16 seed=1 BT=20 sequential",
                    "trials_per_step": t, "tier": {1: "hbm", 2: "lds"}.get(agg["tier"], "auto"),
-                   "layout": args.layout,
+                   "layout": args.layout, "layout_used": layout_name,
                    "trials_per_pass": agg["chunk"]},
         "insertions_per_s": value * p.pnum,
         "probes_per_s": agg["probes"] / elapsed,
         "probes_per_trial": probes_per_trial,
         "failed": agg["failed"],
         "roofline": {
-            "kernel": "k_chain<3,bgh> (insertion chains, one lane per trial)",
+            "kernel": (prof["kernel"] if prof else layout_name) + " (insertion chains, one lane per trial)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "avg_launch_ms": chain_avg_s * 1e3, "probes_per_launch": probes_per_launch,
@@ -677,12 +680,14 @@ def main():
             "achieved_note": "8 algorithmic bytes per probe: one u32 slot of the reference's table read and at "
                              "most one written",
             "own_slot_bytes_per_launch": own_bytes_per_launch,
-            "own_slot_gbs": own_bytes_per_launch / chain_avg_s / 1e9 if chain_avg_s > 0 else None,
-            "own_slot_note": "this build's descriptor layout: 16 B per probe (an 8-byte slot read and written "
-                             "back) and 8 B per inserted item (its descriptor)",
-            "traffic_source": "profile-derived, not measured in this run: (FETCH_SIZE + WRITE_SIZE) per probe "
-                              "of k_chain from rocprofv3 --pmc (profiles/k_chain_hbm_n16384_desc_traffic.md) "
-                              "times this run's probes per launch",
+            "own_slot_gbs": own_bytes_per_launch / chain_avg_s / 1e9 if own_bytes_per_launch and chain_avg_s > 0
+            else None,
+            "own_slot_note": (f"this build's {layout_name} layout: {prof['slot_bytes']} B per probe (its slot read "
+                              f"and written back) and {prof['item_bytes']} B per inserted item (its candidate word)"
+                              if prof else None),
+            "traffic_source": ("profile-derived, not measured in this run: (FETCH_SIZE + WRITE_SIZE) per probe from "
+                               f"rocprofv3 --pmc ({prof['profile']}) times this run's probes per launch"
+                               if traffic else None),
             "traffic_rate_gbs": traffic / chain_avg_s / 1e9 if traffic and chain_avg_s > 0 else None,
         },
         "int_ops_model": {
@@ -691,7 +696,8 @@ def main():
         },
         "device_ms": {"build": agg["build_ms"], "chain": agg["chain_ms"]},
     }
-    rec["roofline"]["random_access_ceiling"] = random_access_ceiling(probes_per_launch / chain_avg_s)
+    rec["roofline"]["random_access_ceiling"] = random_access_ceiling(probes_per_launch / chain_avg_s,
+                                                                     prof["requests"] if prof else None)
     ok = True
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)

@@ -21,6 +21,8 @@
 // passes in order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -114,8 +116,11 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build_mix(const SegDev* 
 //   LAY_DESC u64 packed candidate stream, u64 entries candidates | choice;
 //   LAY_RES  u64 packed candidate stream, u64 entries holding the occupant's
 //            other k-1 candidates | choice (its current bucket is the slot):
-//            one random read per probe where the descriptor is too wide.
-constexpr uint32_t LAY_MAP = 0, LAY_DESC = 1, LAY_RES = 2;
+//            one random read per probe where the descriptor is too wide;
+//   LAY_RES32 the same entries in 32 bits where they fit: half the
+//            descriptor's table, so a pass's tables take less memory and
+//            address space.
+constexpr uint32_t LAY_MAP = 0, LAY_DESC = 1, LAY_RES = 2, LAY_RES32 = 3;
 
 // chain: one wave per segment, one lane per trial.
 template <int K, bool BGH, uint32_t LAY>
@@ -140,10 +145,11 @@ __global__ __launch_bounds__(256) void k_chain_mix(const SegDev* __restrict__ se
                 failed = chain_bgh_desc<K>(S, D, m, t, walks[slot], rnds[slot], pr);
             else
                 failed = chain_drrt_desc<K>(S, D, m, t, pr);
-        } else if constexpr (LAY == LAY_RES) {
+        } else if constexpr (LAY == LAY_RES || LAY == LAY_RES32) {
+            using E = std::conditional_t<LAY == LAY_RES32, uint32_t, uint64_t>;
             const uint64_t* m = reinterpret_cast<const uint64_t*>(arena + G.map_off + size_t(lane) * G.map_bytes);
-            uint64_t* t = reinterpret_cast<uint64_t*>(arena + G.table_off + size_t(lane) * G.table_bytes);
-            const ResCodec<K, uint64_t> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+            const auto t = res_table<E>(arena + G.table_off + size_t(lane) * G.table_bytes);
+            const ResCodec<K, typename ResWord<E>::type> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
             const CandStream<K, uint64_t> st{m, S.dbits, (1ull << S.dbits) - 1};
             uint32_t stash_left = S.stash;
             if constexpr (BGH) {
@@ -279,7 +285,7 @@ static int set_fail(Engine& E, const std::string& msg) {
 }
 
 static uint32_t group_of(const Job& j) {
-    return (j.spec.p.k << 4) | (j.bgh ? 8u : 0u) | (j.lay << 1) | (j.cached ? 1u : 0u);
+    return (j.spec.p.k << 8) | (j.bgh ? 0x80u : 0u) | (j.lay << 1) | (j.cached ? 1u : 0u);
 }
 
 // Slots of job j that fit in `slots_left` slots and `arena_left` bytes,
@@ -451,12 +457,16 @@ static void launch_chain(Engine& E, const PassBuf& B, bool bgh, uint32_t lay) {
         CH(true, LAY_DESC);
     else if (bgh && lay == LAY_RES)
         CH(true, LAY_RES);
+    else if (bgh && lay == LAY_RES32)
+        CH(true, LAY_RES32);
     else if (bgh)
         CH(true, LAY_MAP);
     else if (lay == LAY_DESC)
         CH(false, LAY_DESC);
     else if (lay == LAY_RES)
         CH(false, LAY_RES);
+    else if (lay == LAY_RES32)
+        CH(false, LAY_RES32);
     else
         CH(false, LAY_MAP);
 #undef CH
@@ -766,12 +776,27 @@ int ck_engine_submit(ck_engine* h, const ck_job* job, uint64_t* job_id) {
     j->S = make_shape(job->p);
     j->cached = job->p.half_bits <= kMaxCacheHalf &&
                 tab_bytes(kRepl) + (size_t(8) << job->p.half_bits) * 2 <= E.lds_block;
-    const uint32_t kb = job->p.k * j->S.dbits;
-    j->lay = kb + 2 <= 63 ? LAY_DESC : (kb - j->S.dbits + 2 <= 63 && kb <= 64) ? LAY_RES : LAY_MAP;
+    // Residual entries in 32 bits where they fit (half the descriptor's table;
+    // faster at n = 2^14), then the descriptor (faster than 48-bit residual
+    // entries: 361K vs 329K trials/s at n = 30000, k = 3), then residual
+    // entries in 64 bits, then the bucket map.  A residual entry of `need` bits
+    // must stay below all-ones: fewer bits than its width, or exactly its width
+    // with a choice below 3.
+    const uint32_t kb = job->p.k * j->S.dbits, need = kb - j->S.dbits + 2;
+    const auto fits = [&](uint32_t width) { return need < width || (need == width && job->p.k <= 3); };
+    if (kb <= 64 && fits(32))
+        j->lay = LAY_RES32;
+    else if (kb + 2 <= 63)
+        j->lay = LAY_DESC;
+    else if (kb <= 64 && fits(64))
+        j->lay = LAY_RES;
+    else
+        j->lay = LAY_MAP;
     j->bgh = job->p.mode == CK_MODE_BGH25;
     j->group = group_of(*j);
+    const size_t entry = j->lay == LAY_RES32 || j->lay == LAY_MAP ? 4 : 8;
     j->map_bytes = align256(size_t(job->p.pnum) * (j->lay != LAY_MAP ? 8 : size_t(job->p.k) * 4));
-    j->table_bytes = align256(size_t(job->p.buckets) * (j->lay != LAY_MAP ? 8 : 4));
+    j->table_bytes = align256(size_t(job->p.buckets) * entry);
     if ((j->map_bytes + j->table_bytes) * SEG + 1024 > E.arena_per)
         return fail(CK_ENOMEM, "64 trials of this shape do not fit one pass buffer");
     j->bw = build_weight(job->p, j->cached);

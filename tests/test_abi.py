@@ -106,11 +106,14 @@ def test_plan_describe_follows_the_placement_request():
     AES as asked, the LDS tier's map or round-cache form, the budget."""
     p = ck.params("bgh25", 16384, 16, kick=500)
     pl = ck.plan_describe(p, 1 << 17)
-    assert (pl.tier, pl.layout, pl.round_cache) == (ck.TIER_HBM, ck.LAYOUT_DESC_PLAIN, 1)
+    assert (pl.tier, pl.layout, pl.round_cache) == (ck.TIER_HBM, ck.LAYOUT_RESIDUAL, 1)  # 2 x 15 + 2 bits: u32
     assert pl.trials_per_pass == 65536 and pl.buffers == 2 and pl.chain_waves == 4 and pl.chain_lds_bytes == 0
     assert ck.plan_describe(p, 1 << 17, prp=ck.PRP_DIRECT).round_cache == 0
     small = ck.plan_describe(p, 1 << 17, budget_bytes=256 << 20)
-    assert small.workspace_bytes <= 256 << 20 and 0 < small.trials_per_pass < 1024
+    # A 256 MiB budget caps the pass: one buffer, the items in 64 windows.
+    assert small.workspace_bytes <= 256 << 20 and 0 < small.trials_per_pass < 4096
+    assert small.buffers == 1 and small.windows == 64
+    assert ck.plan_describe(ck.params("bgh25", 30000, 16, buckets=40000), 1024).layout == ck.LAYOUT_DESC_PLAIN
     wide = ck.params("bgh25", 16384, 32)  # half_bits 17: no round cache fits
     assert ck.plan_describe(wide, 1024).round_cache == 0
     with pytest.raises(ck.CkError):

@@ -69,9 +69,9 @@ def test_reads_reference_journal(tmp_path, fixtures, mode):
 
 
 @pytest.mark.parametrize("extra,enabled,planned", [
-    ([], True, "cached-global-desc-plain"),
-    (["--prp-cache", "off"], False, "direct-global-desc-plain"),
-    (["--prp-cache", "on", "--table-layout", "global"], True, "cached-global-desc-plain"),
+    ([], True, "cached-global-residual"),
+    (["--prp-cache", "off"], False, "direct-global-residual"),
+    (["--prp-cache", "on", "--table-layout", "global"], True, "cached-global-residual"),
     (["--table-layout", "shared", "--shared-variant", "map"], True, "shared-map"),
     (["--table-layout", "shared", "--shared-variant", "round"], True, "shared-trial"),
     (["--tier", "lds"], True, "shared-map"),

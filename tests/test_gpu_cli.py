@@ -163,11 +163,11 @@ def test_cli_checkpoint_resume_n2_20(tmp_path):
 # --prp-cache, --micro, --block-threads, --table-layout, --warps-per-block,
 # --shared-variant, and the table= value the Worker line must show.
 PREFLIGHT = [
-    ("gpu_direct", "off", 257, 128, "global", 0, "auto", "direct-global-desc-plain"),
-    ("gpu_cached128", "on", 257, 128, "global", 0, "auto", "cached-global-desc-plain"),
+    ("gpu_direct", "off", 257, 128, "global", 0, "auto", "direct-global-residual"),
+    ("gpu_cached128", "on", 257, 128, "global", 0, "auto", "cached-global-residual"),
     ("gpu_shared_round", "on", 257, 256, "shared", 0, "round", "shared-trial"),
     ("gpu_shared_map", "on", 257, 256, "shared", 0, "map", "shared-map"),
-    ("gpu_cached_tail", "on", 37, 256, "global", 0, "auto", "cached-global-desc-plain"),
+    ("gpu_cached_tail", "on", 37, 256, "global", 0, "auto", "cached-global-residual"),
     ("gpu_shared_tail", "on", 5, 256, "shared", 2, "map", "shared-map"),
 ]
 

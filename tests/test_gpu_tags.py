@@ -45,6 +45,7 @@ def test_tagged_tables_match_cleared_tables(shape):
     count = 7 * CHUNK - 5  # seven passes, the last one short
     with ck.Context(0) as d, ck.Context(0) as m:
         d.set_chunk(CHUNK)
+        d.set_layout(ck.LAYOUT_DESC_PLAIN)  # the tagged layout (auto prefers u32 residual entries where they fit)
         m.set_chunk(CHUNK)
         m.set_layout(ck.LAYOUT_MAP)
         for call in range(3):
@@ -58,10 +59,10 @@ def test_tagged_tables_match_cleared_tables(shape):
 def test_tables_cleared_where_the_plan_changes():
     a = ck.params("bgh25", 2000, 16, buckets=2600, kick=500, stash=1)
     b = ck.params("drrt18", 3000, 16, k=4, buckets=3600, kick=100)
-    steps = [(a, ck.LAYOUT_AUTO, False, 3000), (a, ck.LAYOUT_AUTO, False, 3000), (b, ck.LAYOUT_AUTO, False, 2500),
-             (a, ck.LAYOUT_AUTO, False, 3000), (a, ck.LAYOUT_MAP, False, 3000), (a, ck.LAYOUT_AUTO, False, 3000),
-             (a, ck.LAYOUT_AUTO, True, 3000), (a, ck.LAYOUT_AUTO, False, 3000), (a, ck.LAYOUT_AUTO, False, 4000),
-             (b, ck.LAYOUT_DESC, False, 2500), (b, ck.LAYOUT_AUTO, False, 2500)]
+    T = ck.LAYOUT_DESC_PLAIN  # tagged tables
+    steps = [(a, T, False, 3000), (a, T, False, 3000), (b, T, False, 2500), (a, T, False, 3000),
+             (a, ck.LAYOUT_MAP, False, 3000), (a, T, False, 3000), (a, T, True, 3000), (a, T, False, 3000),
+             (a, ck.LAYOUT_AUTO, False, 3000), (a, T, False, 4000), (b, ck.LAYOUT_DESC, False, 2500), (b, T, False, 2500)]
     with ck.Context(0) as d:
         d.set_chunk(CHUNK)
         for i, (p, layout, digest, count) in enumerate(steps):
