@@ -52,9 +52,10 @@ enum {
 
 /* HBM-tier table layout and probe form. */
 enum {
-    CK_LAYOUT_AUTO = 0,       /* DESC_PLAIN when it fits and tables are not requested; else RESIDUAL
-                                 when it fits ((k-1)*bits+2 <= 63); else MAP_XCHG below 2^14 trials
-                                 per pass and MAP above */
+    CK_LAYOUT_AUTO = 0,       /* tables not requested: RESIDUAL when its entries fit 32 bits
+                                 ((k-1)*bits+2 <= 32) and k*bits <= 64; else DESC_PLAIN when it fits;
+                                 else RESIDUAL when it fits ((k-1)*bits+2 <= 63); otherwise MAP_XCHG
+                                 below 2^14 trials per pass and MAP above */
     CK_LAYOUT_MAP = 1,        /* entry = ordinal | choice (u16 when it fits); load + store probes */
     CK_LAYOUT_DESC = 2,       /* entry = occupant's packed candidates | choice (k*bits+2 <= 63); exchange probes */
     CK_LAYOUT_MAP_XCHG = 3,   /* u32 entry = ordinal | choice; exchange probes */
