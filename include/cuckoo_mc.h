@@ -61,8 +61,8 @@ enum {
     CK_LAYOUT_MAP_XCHG = 3,   /* u32 entry = ordinal | choice; exchange probes */
     CK_LAYOUT_DESC_PLAIN = 4, /* descriptor entries; load + store probes */
     CK_LAYOUT_RESIDUAL = 5    /* entry = occupant's other k-1 candidates | choice (u32 when it fits, else
-                                 48 bits in 6 bytes when it fits, else u64), its current bucket implied by
-                                 the slot; load + store probes */
+                                 44 bits packed end to end, else 48 bits in 6 bytes, else u64), its current
+                                 bucket implied by the slot; load + store probes */
 };
 
 typedef struct ck_stats {

@@ -218,8 +218,9 @@ __device__ __forceinline__ bool chain_drrt_desc(const Shape& S, const DescCodec&
 // the entry it just read and the slot it read it from.  (k-1)*bb + 2 bits fit
 // a u32 entry at n = 2^14, k = 3 (2*15 + 2 = 32, choice <= 2 keeps the entry
 // below all-ones) and a u64 entry up to n = 2^20 and beyond, where the full
-// descriptor (k*bb + 2 bits) no longer fits.  Between the two, 48-bit entries
-// (Entries48) take 6 bytes: n = 2^20, k = 3 needs 2*21 + 2 = 44 bits.  New
+// descriptor (k*bb + 2 bits) no longer fits.  Between the two, 44-bit entries
+// (Entries44) take 5.5 bytes -- n = 2^20, k = 3 needs 2*21 + 2 = 44 bits -- and
+// 48-bit entries (Entries48) 6 bytes.  New
 // items take their candidates from a stream in insertion order: one packed u64
 // per item when k*bb <= 64, else the item's k map entries; it is prefetched
 // one item ahead.
@@ -240,10 +241,10 @@ struct Entries48 {
         char* a;
         mutable uint64_t raw = 0;  // the 8 bytes read, once read
         mutable bool read = false;
-        __device__ __forceinline__ uint32_t mis() const {
+        __host__ __device__ __forceinline__ uint32_t mis() const {
             return static_cast<uint32_t>(reinterpret_cast<uintptr_t>(a)) & 3u;
         }
-        __device__ __forceinline__ operator uint64_t() const {
+        __host__ __device__ __forceinline__ operator uint64_t() const {
             // pointer arithmetic, not an integer round trip, keeps the global address space
             const uint32_t m = mis();
             const uint32_t* w = reinterpret_cast<const uint32_t*>(a - m);
@@ -252,7 +253,7 @@ struct Entries48 {
             const uint64_t v = (raw >> (m * 8)) & 0xFFFFFFFFFFFFull;
             return v == 0xFFFFFFFFFFFFull ? ~0ull : v;
         }
-        __device__ __forceinline__ void operator=(uint64_t v) const {
+        __host__ __device__ __forceinline__ void operator=(uint64_t v) const {
             const uint32_t m = mis();
             if (read) {
                 const uint64_t keep = ~(0xFFFFFFFFFFFFull << (m * 8));
@@ -269,15 +270,72 @@ struct Entries48 {
             }
         }
     };
-    __device__ __forceinline__ Ref operator[](uint32_t b) const { return Ref{base + size_t(b) * 6}; }
+    __host__ __device__ __forceinline__ Ref operator[](uint32_t b) const { return Ref{base + size_t(b) * 6}; }
 };
 
-// Residual table entry types of the kernels: u32, u64, or E48 (Entries48,
-// decoded as u64).
+// A slot's table of 44-bit residual entries packed end to end (5.5 bytes per
+// entry): entry b is bits [44b, 44b + 44) of the table read as little-endian
+// dwords from a 4-byte aligned base.  44b is a multiple of 4, so the entry
+// starts at bit off = 44b mod 32, one of 0, 4, ..., 28, of dword 44b >> 5, and
+// spans two dwords when off <= 20, three when off is 24 or 28 (2 entries in
+// 8).  A read loads exactly the dwords the entry spans (two as one 8-byte
+// load) and maps the 44-bit all-ones (empty) to the u64 all-ones.  A write
+// stores the same dwords back with the entry's bits replaced and the
+// neighbours' bits as they were read; as with Entries48 that is safe because
+// one lane owns a table and nothing else writes it between the probe's read
+// and its write.  A write before any read reads first.
+struct Entries44 {
+    static constexpr uint64_t kMask = (1ull << 44) - 1;
+    char* base;
+    struct Ref {
+        uint32_t* w;
+        uint32_t off;
+        mutable uint64_t lo = 0;  // the entry's first two dwords, once read
+        mutable uint32_t hi = 0;  // its third dword (off > 20), once read
+        mutable bool read = false;
+        __host__ __device__ __forceinline__ bool three() const { return off > 20; }
+        __host__ __device__ __forceinline__ void load() const {
+            lo = w[0] | (uint64_t(w[1]) << 32);
+            hi = three() ? w[2] : 0u;
+            read = true;
+        }
+        __host__ __device__ __forceinline__ operator uint64_t() const {
+            load();
+            const uint64_t top = three() ? uint64_t(hi) << (64 - off) : 0ull;
+            const uint64_t v = ((lo >> off) | top) & kMask;
+            return v == kMask ? ~0ull : v;
+        }
+        __host__ __device__ __forceinline__ void operator=(uint64_t v) const {
+            if (!read) load();
+            v &= kMask;
+            const uint64_t x = (lo & ~(kMask << off)) | (v << off);
+            w[0] = static_cast<uint32_t>(x);
+            w[1] = static_cast<uint32_t>(x >> 32);
+            if (three()) {
+                const uint32_t hm = static_cast<uint32_t>(kMask >> (64 - off));  // the entry's bits in dword 2
+                w[2] = (hi & ~hm) | static_cast<uint32_t>(v >> (64 - off));
+            }
+        }
+    };
+    __host__ __device__ __forceinline__ Ref operator[](uint32_t b) const {
+        const uint64_t bit = uint64_t(b) * 44;
+        return Ref{reinterpret_cast<uint32_t*>(base) + (bit >> 5), static_cast<uint32_t>(bit & 31)};
+    }
+    // Table bytes for m entries: whole dwords, plus 8 bytes of slack.
+    static constexpr size_t bytes(size_t m) { return (m * 44 + 31) / 32 * 4 + 8; }
+};
+
+// Residual table entry types of the kernels: u32, u64, E44 (Entries44) or E48
+// (Entries48), the last two decoded as u64.
+struct E44 {};
 struct E48 {};
 template <class E>
 struct ResWord {
     using type = E;
+};
+template <>
+struct ResWord<E44> {
+    using type = uint64_t;
 };
 template <>
 struct ResWord<E48> {
@@ -287,6 +345,8 @@ template <class E>
 __device__ __forceinline__ auto res_table(char* p) {
     if constexpr (std::is_same<E, E48>::value)
         return Entries48{p};
+    else if constexpr (std::is_same<E, E44>::value)
+        return Entries44{p};
     else
         return reinterpret_cast<E*>(p);
 }
@@ -346,7 +406,7 @@ struct ItemSpan {
 };
 
 // BGH25 over the residual layout (the decisions of cuckoo::insert_core_k).
-// T is the table: an E* or Entries48 (E = u64).
+// T is the table: an E* or Entries44 / Entries48 (E = u64).
 template <int K, class E, class M, class T>
 __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, E>& R, const CandStream<K, M>& st,
                                               T t, ItemSpan span, uint64_t& walk, uint64_t& rnd,

@@ -280,6 +280,18 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The chain kernels' block order.  The dispatcher hands workgroup i to XCD
+// i mod 8, and each XCD translates addresses through its own TLBs.  xcd_block
+// turns the dispatch index b of n blocks into a block index such that each XCD
+// runs one contiguous range of blocks (XCDs 0 .. n mod 8 - 1 get one block more
+// than the rest), so the trial slots of one XCD -- and their tables, laid out
+// in slot order -- span one eighth of the pass's memory instead of all of it.
+// It is a bijection on [0, n).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, j = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
 // Failure counting: one 64-bit ballot per wave, one atomic per wave with a
 // failure, each failing lane writes at base + its rank among failing lanes.
 __device__ __forceinline__ void append_failures(bool failed, uint32_t offset, uint32_t* __restrict__ count,

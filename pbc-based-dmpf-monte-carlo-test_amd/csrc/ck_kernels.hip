@@ -184,12 +184,16 @@ __global__ __launch_bounds__(256) void k_digest(Shape S, uint32_t count, const M
 }
 
 // chain (HBM tier, residual layout): M is the candidate stream (u64 packed
-// descriptor or u32 map rows), E the residual table entry (u32, u64 or E48).  The
+// descriptor or u32 map rows), E the residual table entry (u32, u64, E44 or E48).  The
 // launch inserts items [item0, item1), whose candidates the stream holds.
 // With `state` (a trial split into windows of items), the chain resumes from
 // and saves, per slot: the walk and random word, state = stash left | failed
 // << 31, and probes_out = probes so far; a failed trial is skipped by later
-// windows.  Without it, item0 = 0 and item1 = pnum.
+// windows.  Without it, item0 = 0 and item1 = pnum.  With `xcd_order` the
+// blocks take their slots in XCD order (xcd_block): measured at n = 2^20, where
+// a windowed pass's tables span about 270 GiB, 8,184 -> 8,355 trials/s (two
+// interleaved pairs); at n = 2^14 (6.4 GB of tables) it was 0.2 % slower, so
+// only windowed passes use it.
 template <int K, bool BGH, class M, class E>
 __global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint32_t count,
                                                    const M* __restrict__ map, size_t map_stride,
@@ -197,8 +201,8 @@ __global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint
                                                    uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
                                                    uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
                                                    uint32_t* __restrict__ probes_out, uint32_t item0, uint32_t item1,
-                                                   uint32_t* __restrict__ state) {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   uint32_t* __restrict__ state, uint32_t xcd_order) {
+    const uint32_t slot = (xcd_order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x) * blockDim.x + threadIdx.x;
     bool failed = false;
     uint32_t pr = 0;
     if (slot < count) {
@@ -634,6 +638,7 @@ static bool res_fits(const ck_params& p, uint32_t width) {
 }
 static bool res_ok(const ck_params& p) { return res_fits(p, 64); }
 static bool res_e32(const ck_params& p) { return res_fits(p, 32); }
+static bool res_e44(const ck_params& p) { return res_fits(p, 44); }
 static bool res_e48(const ck_params& p) { return res_fits(p, 48); }
 static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits <= 64; }
 
@@ -646,7 +651,7 @@ static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits 
 // chain of window j reads the other.
 struct HbmPlan {
     bool desc, map16, table16, xchg;
-    bool res, stream64, e32, e48;  // residual layout: u64 candidate stream, u32 / 48-bit entries
+    bool res, stream64, e32, e44, e48;  // residual layout: u64 candidate stream, u32 / 44-bit / 48-bit entries
     uint32_t window;     // items per map (pnum unless windowed)
     uint32_t nwin;       // windows per trial
     size_t map_stride;   // map entries per slot
@@ -665,7 +670,8 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uin
     h.res = layout == CK_LAYOUT_RESIDUAL;
     h.stream64 = h.res && res_stream64(p);
     h.e32 = h.res && res_e32(p);
-    h.e48 = h.res && !h.e32 && res_e48(p);
+    h.e44 = h.res && !h.e32 && res_e44(p);
+    h.e48 = h.res && !h.e32 && !h.e44 && res_e48(p);
     h.map16 = !desc && !h.res && map16_ok(p);
     h.table16 = !desc && !h.res && !h.xchg && table16_ok(p);
     const bool m64 = desc || h.stream64;
@@ -673,7 +679,8 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uin
     const size_t tw = h.res ? (h.e32 ? 4 : h.e48 ? 6 : 8) : desc ? 8 : h.table16 ? 2 : 4;
     h.map_bytes = align256(size_t(window) * (m64 ? 1 : p.k) * mw);
     h.map_stride = h.map_bytes / mw;
-    h.table_bytes = align256(size_t(p.buckets) * tw + (h.e48 ? 8 : 0));  // 48-bit reads reach 2 bytes past
+    h.table_bytes = h.e44 ? align256(Entries44::bytes(p.buckets))
+                          : align256(size_t(p.buckets) * tw + (h.e48 ? 8 : 0));  // 48-bit reads reach 2 bytes past
     size_t o = 0;
     h.off_cnt = o;
     o = align256(o + CNT_WORDS * 4);
@@ -734,9 +741,11 @@ static uint32_t hbm_chunk(const Ctx& c, const ck_params& p, uint32_t count, uint
 // more throughput.  Measured at n = 2^20 with u64 entries (trials/s): no
 // windows 3,425 (15,137 chains per pass); 8 windows 5,346; 16: 5,813; 32: 6,089
 // (25,552 chains, the same 4.1 s of chain per pass).  With 48-bit entries:
-// 32 windows 6,960 (33,556 chains, 4.75 s of chain per pass); 64: 7,146.  Each
-// window's build derives the trial and its round cache again, under the
-// previous window's chain.
+// 32 windows 6,960 (33,556 chains, 4.75 s of chain per pass); 64: 7,146 (8,051
+// once each 48-bit write became one 8-byte store).  With 44-bit entries and 64
+// windows: 37,614 chains, 8,206-8,231 against 8,051-8,061 for 48-bit entries
+// (interleaved runs).  Each window's build derives the trial and its round
+// cache again, under the previous window's chain.
 static constexpr uint32_t kWindows = 64;
 static uint32_t stream_window(const Ctx& c, const ck_params& p, uint32_t count, uint32_t layout, bool allowed) {
     if (!allowed || layout != CK_LAYOUT_RESIDUAL) return p.pnum;
@@ -887,6 +896,9 @@ static void launch_digest_k(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (h.res && h.e32)
         st(&k_digest_res<K, uint32_t>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
            a.table_bytes, a.digests);
+    else if (h.res && h.e44)
+        st(&k_digest_res<K, E44>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
+           a.table_bytes, a.digests);
     else if (h.res && h.e48)
         st(&k_digest_res<K, E48>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
            a.table_bytes, a.digests);
@@ -910,18 +922,22 @@ static void launch_chain_res_t(const PassArgs& a, Dispatch& st) {
     const uint32_t blocks = (a.count + 255) / 256;
     st(&k_chain_res<K, BGH, M, E>, dim3(blocks), dim3(256), 0, a.S, a.first, a.count,
        reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.cnt, a.fail,
-       a.probes, a.item0, a.item1, a.state);
+       a.probes, a.item0, a.item1, a.state, a.state ? 1u : 0u);
 }
 template <int K, bool BGH>
 static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
     if (h.res && h.stream64 && h.e32)
         launch_chain_res_t<K, BGH, uint64_t, uint32_t>(a, st);
+    else if (h.res && h.stream64 && h.e44)
+        launch_chain_res_t<K, BGH, uint64_t, E44>(a, st);
     else if (h.res && h.stream64 && h.e48)
         launch_chain_res_t<K, BGH, uint64_t, E48>(a, st);
     else if (h.res && h.stream64)
         launch_chain_res_t<K, BGH, uint64_t, uint64_t>(a, st);
     else if (h.res && h.e32)
         launch_chain_res_t<K, BGH, uint32_t, uint32_t>(a, st);
+    else if (h.res && h.e44)
+        launch_chain_res_t<K, BGH, uint32_t, E44>(a, st);
     else if (h.res && h.e48)
         launch_chain_res_t<K, BGH, uint32_t, E48>(a, st);
     else if (h.res)

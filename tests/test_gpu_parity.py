@@ -330,12 +330,14 @@ def test_n2_20_one_and_two_buffer_passes_agree(ctx):
 
 RESIDUAL_SHAPES = [
     ("bgh25", 16384, 16, {"kick": 500}, 64),                                   # u32 entries, 2*15 + 2 = 32 bits
-    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50, "stash": 1}, 256),  # 3*10 + 2 = 32 bits, k = 4: 48-bit
-    ("bgh25", 600, 12, {"k": 4, "buckets": 1000, "kick": 50}, 256),             # 48-bit entries
-    ("bgh25", 30000, 16, {"buckets": 40000, "kick": 300, "stash": 1}, 24),      # 2*16 + 2 = 34 bits: 48-bit
+    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50, "stash": 1}, 256),  # 3*10 + 2 = 32 bits, k = 4: 44-bit
+    ("bgh25", 600, 12, {"k": 4, "buckets": 1000, "kick": 50}, 256),             # 44-bit entries
+    ("bgh25", 30000, 16, {"buckets": 40000, "kick": 300, "stash": 1}, 24),      # 2*16 + 2 = 34 bits: 44-bit
+    ("bgh25", 20000, 16, {"k": 4, "buckets": 30000, "kick": 200, "stash": 1}, 24),  # 3*15 + 2 = 47 bits: 48-bit
+    ("drrt18", 20000, 16, {"k": 4, "buckets": 30000, "kick": 60}, 24),             # 48-bit entries
     ("bgh25", 60000, 16, {"k": 4, "buckets": 70000, "kick": 200, "stash": 2}, 24),  # 4*17 > 64: u32 map stream, u64
     ("drrt18", 60000, 16, {"k": 4, "buckets": 70000, "kick": 100}, 24),            # 3*17 + 2 = 53 bits: u64 entries
-    ("bgh25", 1 << 20, 20, {"buckets": 1363149, "kick": 1000, "stash": 2}, 6),   # n = 2^20: 48-bit entries
+    ("bgh25", 1 << 20, 20, {"buckets": 1363149, "kick": 1000, "stash": 2}, 6),   # n = 2^20: 2*21 + 2 = 44 bits
 ]
 
 

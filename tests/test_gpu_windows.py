@@ -20,9 +20,10 @@ SHAPES = [
     # mode, pnum, ell_i, overrides, trials
     ("bgh25", 1024, 16, {"kick": 500}, 256),                                        # u32 entries, u64 stream
     ("bgh25", 300, 12, {"k": 2, "buckets": 600, "kick": 30, "stash": 2}, 512),       # 162 failures; stash across windows
-    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50, "stash": 1}, 256),    # 48-bit entries
+    ("drrt18", 600, 12, {"k": 4, "buckets": 1000, "kick": 50, "stash": 1}, 256),    # 44-bit entries
     ("drrt18", 200, 10, {"buckets": 230, "kick": 8, "stash": 3}, 512),
-    ("bgh25", 30000, 16, {"buckets": 40000, "kick": 300, "stash": 1}, 16),          # 48-bit entries
+    ("bgh25", 30000, 16, {"buckets": 40000, "kick": 300, "stash": 1}, 16),          # 44-bit entries
+    ("bgh25", 20000, 16, {"k": 4, "buckets": 30000, "kick": 200, "stash": 1}, 16),  # 48-bit entries
     ("bgh25", 60000, 16, {"k": 4, "buckets": 70000, "kick": 200, "stash": 2}, 12),  # u32 map-row stream, u64 entries
 ]
 
