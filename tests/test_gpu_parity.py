@@ -166,6 +166,43 @@ def test_headline_shape_prefix(ctx, tier):
     assert (probes == oprobes).all()
 
 
+HEADLINE_LAYOUTS = {"auto": ck.LAYOUT_AUTO, "residual": ck.LAYOUT_RESIDUAL, "desc-plain": ck.LAYOUT_DESC_PLAIN,
+                    "map": ck.LAYOUT_MAP, "map-xchg": ck.LAYOUT_MAP_XCHG}
+
+
+@pytest.mark.parametrize("kick", [48, 64])
+def test_headline_shape_failure_detection(ctx, kick):
+    """The headline shape (n = 2^14, m = 24576, k = 3; seed 1, batch 0) with a
+    kick bound low enough that trials fail -- about half of them at 48, a few
+    at 64 -- so that failure detection is pinned at this size, not only the
+    absence of failures.  Failing offsets and per-trial probes equal the
+    oracle's under every HBM layout (auto is u32 residual entries), on the LDS
+    tier and through the engine."""
+    p, op = both("bgh25", 16384, 16, kick=kick)
+    count = 256
+    of, oprobes, _ = ock.run_range(ock.batch_key(1, 0), 0, count, op, probes=True)
+    assert 0 < len(of) < count
+    for name, lay in HEADLINE_LAYOUTS.items():
+        ctx.set_layout(lay)
+        try:
+            f, probes, _, _ = ctx.launch_debug(p, ck.batch_key(1, 0), 0, count)
+            g, st = ctx.launch(p, ck.batch_key(1, 0), 0, count)
+        finally:
+            ctx.set_layout(ck.LAYOUT_AUTO)
+        assert list(f) == list(of) == list(g), name
+        assert (probes == oprobes).all() and st.probes == int(oprobes.sum()), name
+    ctx.set_tier(ck.TIER_LDS)
+    try:
+        f, probes, _, st = ctx.launch_debug(p, ck.batch_key(1, 0), 0, count)
+    finally:
+        ctx.set_tier(ck.TIER_AUTO)
+    assert st.tier == ck.TIER_LDS and list(f) == list(of) and (probes == oprobes).all()
+    with ck.Engine(0, arena_bytes=8 << 30) as eng:
+        jid = eng.submit(p, 1, count, bt=20, keep_failures=True)
+        res = eng.wait(jid)
+        assert list(map(int, eng.failures(jid))) == list(of) and res.probes == int(oprobes.sum())
+
+
 def test_headline_shape_whole_batch0(ctx):
     """All 2^20 trials of batch 0 at the headline shape (seed 1).  With the
     reference's own kick bound (3n = 49152) the failing offsets equal those
