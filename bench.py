@@ -57,6 +57,10 @@ CHAIN_PROFILES = {
                      read=65.22, write=27.45, requests=1.0185 + 0.8571,
                      profile="profiles/k_chain_res_hbm_n16384_u32_traffic.md"),
 }
+# The n = 2^20 chain (44-bit residual entries, windowed): HBM bytes and DRAM
+# requests per probe and the L2 hit rate, from its counter profile.
+N20_PROFILE = dict(kernel="k_chain_res<3, bgh, u64 candidate stream, 44-bit entries>", read=57.4, write=30.4,
+                   requests=0.897 + 0.885, l2_hit_rate=0.686, profile="profiles/k_chain_res_hbm_n2_20_counters.md")
 # DRAM requests per step of ck_randbench (profiles/randbench_192k_traffic.md).
 RB_READ_WRITE_REQ_PER_STEP = 0.998 + 1.0005
 RB_READ_ONLY_REQ_PER_STEP = 0.998
@@ -530,6 +534,13 @@ def hbm_leg(ck, trials: int, layout: int = 0, warm: int = 0):
         "algorithmic_gbs": (ALGO_PROBE_BYTES * st.probes + ALGO_BUCKET_INIT_BYTES * p.buckets * trials) / dt / 1e9,
         "chain_algorithmic_gbs": ALGO_PROBE_BYTES * st.probes / chain_s / 1e9 if chain_s > 0 else None,
         "device_ms": {"build": st.build_ms, "chain": st.chain_ms},
+        # HBM traffic and L2 hit rate of the chains: per-probe figures of the
+        # counter profile times this run's probes, over the chains' device time.
+        "chain_hbm_gbs": (N20_PROFILE["read"] + N20_PROFILE["write"]) * st.probes / chain_s / 1e9 if chain_s > 0 else None,
+        "chain_dram_requests_per_s": N20_PROFILE["requests"] * st.probes / chain_s if chain_s > 0 else None,
+        "chain_l2_hit_rate": N20_PROFILE["l2_hit_rate"],
+        "traffic_source": "profile-derived, not measured in this run: per-probe FETCH_SIZE + WRITE_SIZE, "
+                          "TCC_EA0_RDREQ + WRREQ and TCC_HIT / (HIT + MISS) of " + N20_PROFILE["profile"],
     }
 
 
