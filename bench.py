@@ -23,6 +23,14 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "tests"))
 
+T_START = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    """Progress line on stderr, stamped with the seconds since the process started."""
+    print(f"[{time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 METRIC = "cuckoo trials/sec at n=2^14,k=3 (bit-exact); sweep jobs/sec + p50/p99"
 BT = 20
 SEED = 1
@@ -262,8 +270,7 @@ def wait_all(eng, ids, label, t0):
             time.sleep(0.005)
         if time.perf_counter() - last > 20:
             last = time.perf_counter()
-            print(f"{label}: {len(ids) - left} of {len(ids)} jobs done at {last - t0:.1f} s", file=sys.stderr,
-                  flush=True)
+            log(f"{label}: {len(ids) - left} of {len(ids)} jobs done at {last - t0:.1f} s")
     return res
 
 
@@ -290,8 +297,7 @@ def sweep_run(ck, pts, trials_of, plain_too, label):
     plain = plain_s = busy_s = plain_probes = None
     if plain_too:
         plain, plain_s, busy_s, plain_probes = plain_loop(ck, pts, ps, trials_of)
-        print(f"{label}: per-point launches {plain_s:.2f} s (device busy {busy_s:.2f} s)", file=sys.stderr,
-              flush=True)
+        log(f"{label}: per-point launches {plain_s:.2f} s (device busy {busy_s:.2f} s)")
     eng = ck.Engine(0, policy=ck.SCHED_SRPT)
     t0 = time.perf_counter()
     ids = [eng.submit(p, pt["seed"], trials_of(pt), bt=pt["bt"]) for pt, p in zip(pts, ps)]
@@ -299,7 +305,7 @@ def sweep_run(ck, pts, trials_of, plain_too, label):
     eng_s = time.perf_counter() - t0
     st = eng.stats()
     eng.close()
-    print(f"{label}: engine {eng_s:.2f} s", file=sys.stderr, flush=True)
+    log(f"{label}: engine {eng_s:.2f} s")
     equal = plain is None or all((r.failed, r.probes) == pl for r, pl in zip(res, plain))
     total = sum(trials_of(pt) for pt in pts)
     probes = sum(r.probes for r in res)
@@ -345,17 +351,20 @@ def sweep_full_leg(ck, points: int, trials: int = 10**6):
             **rec}, equal
 
 
-def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float, plain_jobs: int):
+def serve_leg(ck, rates, max_jobs: int, budget_lo: float, budget_hi: float, plain_jobs: int, window_s: float):
     """Poisson arrivals (seed 3) of sweep-shaped jobs: the engine (continuous
-    batching, least-remaining-work first) over `jobs` arrivals per rate, and a
-    plain loop that launches one job at a time in arrival order over the first
+    batching, least-remaining-work first) over the arrivals of `window_s`
+    seconds per rate (at least 200 and at most `max_jobs` jobs), and a plain
+    loop that launches one job at a time in arrival order over the first
     `plain_jobs` of them.  Latency = completion - arrival."""
     sw = load_sweep()
-    shapes = sw.sweep_points(jobs, seed=3)
-    ps = [sw.params_of(ck, pt) for pt in shapes]
+    shapes_all = sw.sweep_points(max_jobs, seed=3)
+    ps_all = [sw.params_of(ck, pt) for pt in shapes_all]
     out = []
     ok = True
     for rate in rates:
+        jobs = max(1, min(max_jobs, max(200, int(rate * window_s))))
+        shapes, ps = shapes_all[:jobs], ps_all[:jobs]
         arr = sw.arrivals(jobs, rate, seed=3, budget_lo=budget_lo, budget_hi=budget_hi)
         eng = ck.Engine(0, policy=ck.SCHED_SRPT)
         t0 = time.perf_counter()
@@ -389,8 +398,8 @@ def serve_leg(ck, rates, jobs: int, budget_lo: float, budget_hi: float, plain_jo
         out.append({"rate": rate, "jobs": jobs, "jobs_per_s": jobs / span, "p50_ms": pct(lat, 50),
                     "p99_ms": pct(lat, 99), "plain_jobs": n_plain, "plain_jobs_per_s": n_plain / pspan,
                     "plain_p50_ms": pct(plat, 50), "plain_p99_ms": pct(plat, 99), "results_equal_plain": equal})
-        print(f"serve: rate {rate}: {jobs} jobs, p50 {pct(lat, 50):.1f} ms p99 {pct(lat, 99):.1f} ms "
-              f"(plain, first {n_plain}: {pct(plat, 50):.1f} / {pct(plat, 99):.1f})", file=sys.stderr, flush=True)
+        log(f"serve: rate {rate}: {jobs} jobs, p50 {pct(lat, 50):.1f} ms p99 {pct(lat, 99):.1f} ms "
+              f"(plain, first {n_plain}: {pct(plat, 50):.1f} / {pct(plat, 99):.1f})")
     return {"workload": f"Poisson arrivals (seed 3), shapes as the sweep (seed 3), trial budgets log-uniform "
                         f"[{budget_lo:g}, {budget_hi:g}]", "rates": out}, ok
 
@@ -434,7 +443,7 @@ def serve_stated_leg(ck, rate: float, window_s: float, budget_lo: float = 1e5, b
     submitted_trials = sum(b for _, b in arr)
     executed = st.slots  # trial slots run in passes, finished jobs or not
     fin_budgets = sorted(b for (_, b), i in zip(arr, ids) if i in done)
-    print(f"serve-stated: {len(done)} of {len(ids)} jobs finished in {elapsed:.1f} s", file=sys.stderr, flush=True)
+    log(f"serve-stated: {len(done)} of {len(ids)} jobs finished in {elapsed:.1f} s")
     return {
         "workload": f"Poisson arrivals at {rate:g} jobs/s (seed 3) for {window_s:g} s, shapes as the sweep "
                     f"(seed 3), trial budgets log-uniform [{budget_lo:g}, {budget_hi:g}]",
@@ -476,7 +485,7 @@ def graph_leg(ck, p, steps: int, trials: int):
     launch) and small n = 256 launches of 1,024 trials, where host launch work
     weighs most.  Each form runs in its own context after one untimed launch
     (which builds the graph); the failure lists of every launch must agree."""
-    cases = [("headline n=16384, 2^20 trials per launch", p, trials, steps),
+    cases = [("headline n=16384, 2^20 trials per launch", p, trials, min(steps, 6)),
              ("n=1024 m=1536 kick 500, 1e5 trials per launch", ck.params("bgh25", 1024, 16, kick=500), 100000, 20),
              ("n=256 m=384 kick 500, 1024 trials per launch", ck.params("bgh25", 256, 16, kick=500), 1024, 200)]
     out = []
@@ -496,7 +505,7 @@ def graph_leg(ck, p, steps: int, trials: int):
         out.append({"workload": name, "launches": reps, "kernel_by_kernel_trials_per_s": rate["kernels"],
                     "graph_trials_per_s": rate["graph"], "graph_vs_kernels": rate["graph"] / rate["kernels"],
                     "results_equal": fails["kernels"] == fails["graph"]})
-        print(f"graph: {name}: {rate['graph']:.0f} vs {rate['kernels']:.0f} trials/s", file=sys.stderr, flush=True)
+        log(f"graph: {name}: {rate['graph']:.0f} vs {rate['kernels']:.0f} trials/s")
     return out
 
 
@@ -564,7 +573,10 @@ def main():
     ap.add_argument("--sweep-full-points", type=int, default=16,
                     help="points of the sweep run at 10^6 trials each, engine and per-point (0: skip)")
     ap.add_argument("--serve-rates", default="10,100,1000")
-    ap.add_argument("--serve-jobs", type=int, default=1000, help="arrivals per rate at the small budgets")
+    ap.add_argument("--serve-jobs", type=int, default=1000, help="at most this many arrivals per rate at the small "
+                    "budgets")
+    ap.add_argument("--serve-window", type=float, default=30.0,
+                    help="seconds of arrivals per rate at the small budgets (at least 200 jobs)")
     ap.add_argument("--serve-plain-jobs", type=int, default=100,
                     help="arrivals per rate for the one-at-a-time baseline loop")
     ap.add_argument("--serve-budget", default="10,1e5", help="small trial budget range lo,hi (log-uniform)")
@@ -604,6 +616,7 @@ def main():
     p = ck.params(SHAPE["mode"], SHAPE["pnum"], SHAPE["ell"], kick=SHAPE["kick"])
     ctx = ck.Context(0)
     ctx.self_test()
+    log("device context ready (AES known-answer test passed)")
     ctx.set_tier({"auto": ck.TIER_AUTO, "hbm": ck.TIER_HBM, "lds": ck.TIER_LDS}[args.tier])
     if args.chunk:
         ctx.set_chunk(args.chunk)
@@ -616,7 +629,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-        print(f"copy probe: {args.copy_probe_mib} MiB read + written", file=sys.stderr, flush=True)
+        log(f"copy probe: {args.copy_probe_mib} MiB read + written")
 
     # The table layout the library plans for a step (auto picks by shape).
     layout_name = {ck.LAYOUT_MAP: "map", ck.LAYOUT_DESC: "desc", ck.LAYOUT_MAP_XCHG: "map-xchg",
@@ -632,7 +645,7 @@ def main():
 
     for i in range(args.warmup):
         step(i)
-        print(f"warmup step {i} done", file=sys.stderr, flush=True)
+        log(f"warmup step {i} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     agg = dict(failed=0, probes=0, build_ms=0.0, chain_ms=0.0, launches=0, chunk=0, tier=0)
@@ -646,7 +659,7 @@ def main():
         agg["chunk"], agg["tier"] = st.chunk, st.tier
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    print(f"timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
 
     trials = args.steps * t
     value = trials / elapsed
@@ -675,7 +688,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic: per-trial keys AES-derived on device from (seed 1, batch, offset)",
         "config": {"workload": "bgh25 n=16384 m=24576 k=3 stash=0 kick=500 ell_i=16 seed=1 BT=20 sequential",
-                   "trials_per_step": t, "tier": {1: "hbm", 2: "lds"}.get(agg["tier"], "auto"),
+                   "trials_per_step": t, "tier": {1: "hbm", 2: "lds", 3: "lane"}.get(agg["tier"], "auto"),
                    "layout": args.layout, "layout_used": layout_name,
                    "trials_per_pass": agg["chunk"]},
         "insertions_per_s": value * p.pnum,
@@ -712,23 +725,25 @@ def main():
     ok = True
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)
+        log(f"cpu baseline: {rec['cpu_baseline']['value']:.0f} trials/s on {rec['cpu_baseline']['cores']} cores")
     ctx.close()
     if not args.no_tune:
         with ck.Context(0) as tctx:
             names = {0: "auto", 1: "map", 2: "desc", 3: "map-xchg", 4: "desc-plain", 5: "residual"}
             ents = tctx.tune(p, ck.batch_key(SEED, 0), args.tune_sample)
         rec["autotune"] = {"sample_trials": args.tune_sample, "candidates": [
-            {"tier": {1: "hbm", 2: "lds"}.get(e["tier"], "auto"), "layout": names[e["layout"]], "chunk": e["chunk"],
+            {"tier": {1: "hbm", 2: "lds", 3: "lane"}.get(e["tier"], "auto"), "layout": names[e["layout"]], "chunk": e["chunk"],
              "trials_per_s": round(e["trials_per_s"])} for e in ents]}
         best = max(ents, key=lambda e: e["trials_per_s"])
-        print(f"autotune: best tier {best['tier']} layout {names[best['layout']]} chunk {best['chunk']} "
-              f"{best['trials_per_s']:.0f} trials/s", file=sys.stderr, flush=True)
+        log(f"autotune: best tier {best['tier']} layout {names[best['layout']]} chunk {best['chunk']} "
+              f"{best['trials_per_s']:.0f} trials/s")
     if not args.no_cpu:
         rec["config1"], good = config1_leg(ck)
+        log(f"config 1: {rec['config1']['gpu_trials_per_s']:.0f} trials/s")
         ok = ok and good
     if not args.no_hbm:
         rec["hbm_tier"] = hbm_leg(ck, args.hbm_trials, layout)
-        print(f"hbm leg: {rec['hbm_tier']['trials_per_s']:.1f} trials/s", file=sys.stderr, flush=True)
+        log(f"hbm leg: {rec['hbm_tier']['trials_per_s']:.1f} trials/s")
     if not args.no_sweep:
         rec["sweep"], good = sweep_leg(ck, args.sweep_points, args.sweep_trials, not args.sweep_no_plain)
         ok = ok and good
@@ -739,7 +754,7 @@ def main():
     if not args.no_serve:
         lo, hi = (float(x) for x in args.serve_budget.split(","))
         rec["serving"], good = serve_leg(ck, [float(r) for r in args.serve_rates.split(",")], args.serve_jobs, lo, hi,
-                                         args.serve_plain_jobs)
+                                         args.serve_plain_jobs, args.serve_window)
         ok = ok and good
         if args.serve_stated_window > 0:
             rec["serving"]["stated_budgets"] = serve_stated_leg(ck, args.serve_stated_rate, args.serve_stated_window)

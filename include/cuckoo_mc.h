@@ -45,9 +45,12 @@ typedef struct ck_params {
 
 /* Table placement tier for the insertion chains. */
 enum {
-    CK_TIER_AUTO = 0,
+    CK_TIER_AUTO = 0, /* HBM (the fastest tier on every measured shape; ck_ctx_tune measures all) */
     CK_TIER_HBM = 1,  /* one lane per trial, table and bucket map in HBM */
-    CK_TIER_LDS = 2   /* trials per workgroup, table and bucket map in LDS */
+    CK_TIER_LDS = 2,  /* trials per workgroup, table and bucket map in LDS */
+    CK_TIER_LANE = 3  /* one lane per trial, its residual-entry table (<= 32-bit entries) in LDS, its new
+                         items' candidates streamed from HBM: the reference's per-thread shared-memory
+                         bins (cuckoo_bench.cu:217-220, 876-880) */
 };
 
 /* HBM-tier table layout and probe form. */
@@ -174,8 +177,8 @@ int ck_ctx_set_placement(ck_ctx* ctx, const ck_placement* want);
 
 /* What a launch of `count` trials would do. */
 typedef struct ck_plan {
-    uint32_t tier;             /* CK_TIER_HBM or CK_TIER_LDS */
-    uint32_t layout;           /* CK_LAYOUT_* chosen (HBM tier; CK_LAYOUT_AUTO for the LDS tier) */
+    uint32_t tier;             /* CK_TIER_HBM, CK_TIER_LDS or CK_TIER_LANE */
+    uint32_t layout;           /* CK_LAYOUT_* chosen (HBM tier; RESIDUAL for the lane tier; AUTO for the LDS tier) */
     uint32_t round_cache;      /* 1: bucket evaluations read a round cache; 0: direct AES */
     uint32_t lds_variant;      /* CK_LDS_MAP or CK_LDS_ROUND (LDS tier) */
     uint32_t trials_per_pass;  /* HBM: chains per pass; LDS: trials per fused launch */

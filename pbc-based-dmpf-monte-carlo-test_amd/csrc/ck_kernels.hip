@@ -29,6 +29,7 @@
 #include "ck_chain.hpp"
 #include "ck_device.hpp"
 #include "ck_internal.hpp"
+#include "ck_lane.hpp"
 
 namespace ck {
 
@@ -256,6 +257,78 @@ __global__ __launch_bounds__(256) void k_digest_res(Shape S, uint32_t count, con
         sum += slot_digest(b, h, choice);
     }
     out[slot] = sum;
+}
+
+// chain (lane tier): one trial per lane, each of the block's `lanes` lanes
+// (lanes <= 64, one wave per block) holding its whole residual table and the
+// ring of its candidate stream in LDS.  Block LDS: [tables: lanes x buckets
+// entries E, entry b of lane j at element b*lanes + j, rounded up to 16 bytes |
+// rings: kLaneRing u64 words per lane, word i of lane j at i*lanes + j].  The
+// stream holds one packed u64 per item (bits per bucket index S.dbits), rows of
+// map_stride words per slot.  With `digests`, each lane also writes the digest
+// of its final table (k_digest_res over the LDS table).
+template <int K, bool BGH, class E>
+__global__ __launch_bounds__(64) void k_chain_lane(Shape S, uint32_t first, uint32_t count, uint32_t lanes,
+                                                   const uint64_t* __restrict__ map, size_t map_stride,
+                                                   const uint64_t* __restrict__ walks, const uint64_t* __restrict__ rnds,
+                                                   uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
+                                                   uint32_t* __restrict__ probes_out,
+                                                   unsigned long long* __restrict__ digests) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) u32x4* lds_u32x4p;
+    extern __shared__ uint32_t lds[];
+    const uint32_t j = threadIdx.x;
+    const uint32_t slot = blockIdx.x * lanes + j;
+    const bool mine = j < lanes && slot < count;
+    const uint32_t base = lds_offset(lds);
+    const uint32_t tbytes = (lanes * S.buckets * uint32_t(sizeof(E)) + 15u) & ~15u;
+    const u32x4 empty4 = {EMPTY, EMPTY, EMPTY, EMPTY};
+    for (uint32_t i = j; i < tbytes / 16; i += 64)
+        *reinterpret_cast<lds_u32x4p>(static_cast<uintptr_t>(base + i * 16)) = empty4;
+    wave_lds_sync();
+
+    const LaneTable<E> t{base + j * uint32_t(sizeof(E)), lanes * uint32_t(sizeof(E))};
+    LaneFeed<K> f;
+    f.row = (typename LaneFeed<K>::grow)(map + (mine ? size_t(slot) * map_stride : 0));
+    f.ring = base + tbytes + j * 8u;
+    f.ring_stride = lanes * 8u;
+    f.bb = S.dbits;
+    f.bmask = (1ull << S.dbits) - 1;
+    if (mine) {
+        f.issue(0);
+        f.land();
+        f.issue(kLaneChunk);
+        f.land();
+    }
+    const ResCodec<K, E> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
+    uint32_t pr = 0;
+    bool failed;
+    if constexpr (BGH) {
+        uint64_t walk = mine ? walks[slot] : 0, rnd = mine ? rnds[slot] : 0;
+        // Wait for these two loads here: inside the chain loop a wait for a
+        // load issued before the loop would also wait for the refill loads in
+        // flight (the vector memory counter is in order), every iteration.
+        asm volatile("" ::"v"(walk), "v"(rnd));
+        failed = chain_bgh_lane<K, E>(S, R, f, t, mine, walk, rnd, pr);
+    } else
+        failed = chain_drrt_lane<K, E>(S, R, f, t, mine, pr);
+    if (mine && probes_out) probes_out[slot] = pr;
+    if (mine && digests) {
+        uint64_t sum = 0;
+        for (uint32_t b = 0; b < S.buckets; ++b) {
+            const E e = t.get(b);
+            if (e == static_cast<E>(~E(0))) continue;
+            Cands c = {};
+            const uint32_t choice = R.dec(e, b, c);
+            uint64_t h = 0;
+#pragma unroll
+            for (int w = 0; w < K; ++w) h += cand_hash_term(w, c[w]);
+            sum += slot_digest(b, h, choice);
+        }
+        digests[slot] = sum;
+    }
+    append_failures(failed, first + slot, counters + CNT_FAIL, fail_offsets);
+    add_wave_total(pr, reinterpret_cast<unsigned long long*>(counters + CNT_PROBES));
 }
 
 // LDS tier: a persistent wave per trial at a time.  Wave region layout:
@@ -618,13 +691,6 @@ static LdsPlan plan_lds(const ck_params& p, size_t lds_block, size_t lds_cu, boo
     return L;
 }
 
-// The HBM tier is faster than the LDS tier on every measured shape (n = 1024 and
-// n = 2^14, both modes): an LDS chain probe costs two dependent LDS round
-// trips on one lane while the HBM tier keeps ~10^5 chains in flight.  Auto
-// therefore picks HBM; the LDS tier stays selectable.
-static uint32_t choose_tier(const Ctx& c, const ck_params&) {
-    return c.tier != CK_TIER_AUTO ? c.tier : CK_TIER_HBM;
-}
 
 // Descriptor layout fits when k packed bucket indices and a 2-bit choice leave
 // the top bit of a u64 clear (so all-ones stays the empty entry).
@@ -642,6 +708,56 @@ static bool res_e44(const ck_params& p) { return res_fits(p, 44); }
 static bool res_e48(const ck_params& p) { return res_fits(p, 48); }
 static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits <= 64; }
 
+// Lane tier: residual entries of at most 32 bits (16 when they fit) and a u64
+// candidate stream.  A block is one wave with `lanes` of its lanes running
+// trials; the planner takes the lanes per block that put the most trials on a
+// CU (blocks of lanes x (buckets entries + kLaneRing ring words) bytes).
+struct LanePlan {
+    bool ok = false, e16 = false;
+    uint32_t lanes = 0, blocks_per_cu = 0, lanes_per_cu = 0;
+    size_t lds = 0;
+};
+static bool lane_fits(const ck_params& p) { return res_e32(p) && res_stream64(p); }
+static size_t lane_block_bytes(const ck_params& p, uint32_t lanes, size_t eb) {
+    return ((size_t(lanes) * p.buckets * eb + 15) & ~size_t(15)) + size_t(lanes) * kLaneRing * 8;
+}
+static LanePlan plan_lane(const ck_params& p, size_t lds_block, size_t lds_cu) {
+    LanePlan L;
+    if (!lane_fits(p)) return L;
+    L.e16 = res_fits(p, 16);
+    const size_t eb = L.e16 ? 2 : 4;
+    for (uint32_t w = 64; w >= 1; --w) {
+        const size_t blk = lane_block_bytes(p, w, eb);
+        if (blk > lds_block) continue;
+        const uint32_t per_cu = static_cast<uint32_t>(std::min<size_t>(lds_cu / blk, 32));  // 32 waves per CU
+        if (per_cu * w > L.lanes_per_cu) {
+            L.lanes_per_cu = per_cu * w;
+            L.lanes = w;
+            L.blocks_per_cu = per_cu;
+            L.lds = blk;
+        }
+    }
+    L.ok = L.lanes_per_cu > 0;
+    return L;
+}
+// Tier of a launch: auto is the HBM tier, measured faster than both on-chip
+// tiers on every shape tried.
+//  - LDS tier (a wave per trial; n = 1024 and n = 2^14, both modes): an LDS
+//    chain probe costs two dependent LDS round trips on one lane while the
+//    HBM tier keeps ~10^5 chains in flight.
+//  - Lane tier (bgh25, k = 2 and 3, n = 256 .. 4096, m = 1.3n .. 1.5n, one
+//    65,536-trial pass without overlap): its chains ran at 26.3 G probes/s
+//    at n = 256 (80 lanes per CU) down to 5.9 G at n = 2048 (12 lanes per
+//    CU), against 25-41 G for the HBM tier, whose small tables stay in the
+//    L2 and the MALL; with passes overlapped it lost 0.13-0.78x end to end,
+//    because its chain blocks hold the LDS that the build blocks need.  A
+//    lane-tier iteration (one probe per lane) took 580-780 ns against
+//    ~1.3 us for an HBM-tier probe at low load: too few lanes fit on chip.
+// The autotuner measures both on-chip tiers and keeps the fastest placement.
+static uint32_t choose_tier(const Ctx& c, const ck_params&, bool = false) {
+    return c.tier != CK_TIER_AUTO ? c.tier : CK_TIER_HBM;
+}
+
 // HBM-tier pass buffer: [counters | map C x map_bytes (x2 with windows) |
 // tables C x table_bytes | walk C | rnd C | probes C | fail C | keys C x 44 |
 // digests C | state C], twice (double-buffered).  In the descriptor layout the
@@ -649,8 +765,12 @@ static bool res_stream64(const ck_params& p) { return p.k * make_shape(p).dbits 
 // u64.  With item windows (residual layout) the map holds the candidates of
 // `window` items, in two halves: the build of window j+1 fills one while the
 // chain of window j reads the other.
+// Internal layout value of the lane tier: residual entries, tables in LDS.
+static constexpr uint32_t kLayoutLane = 100;
 struct HbmPlan {
     bool desc, map16, table16, xchg;
+    bool lane;           // lane tier: no tables in HBM, k_chain_lane
+    LanePlan lp;
     bool res, stream64, e32, e44, e48;  // residual layout: u64 candidate stream, u32 / 44-bit / 48-bit entries
     uint32_t window;     // items per map (pnum unless windowed)
     uint32_t nwin;       // windows per trial
@@ -667,7 +787,8 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uin
     const bool desc = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_DESC_PLAIN;
     h.desc = desc;
     h.xchg = layout == CK_LAYOUT_DESC || layout == CK_LAYOUT_MAP_XCHG;
-    h.res = layout == CK_LAYOUT_RESIDUAL;
+    h.lane = layout == kLayoutLane;
+    h.res = layout == CK_LAYOUT_RESIDUAL || h.lane;
     h.stream64 = h.res && res_stream64(p);
     h.e32 = h.res && res_e32(p);
     h.e44 = h.res && !h.e32 && res_e44(p);
@@ -679,8 +800,9 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uin
     const size_t tw = h.res ? (h.e32 ? 4 : h.e48 ? 6 : 8) : desc ? 8 : h.table16 ? 2 : 4;
     h.map_bytes = align256(size_t(window) * (m64 ? 1 : p.k) * mw);
     h.map_stride = h.map_bytes / mw;
-    h.table_bytes = h.e44 ? align256(Entries44::bytes(p.buckets))
-                          : align256(size_t(p.buckets) * tw + (h.e48 ? 8 : 0));  // 48-bit reads reach 2 bytes past
+    h.table_bytes = h.lane ? 0
+                   : h.e44 ? align256(Entries44::bytes(p.buckets))
+                           : align256(size_t(p.buckets) * tw + (h.e48 ? 8 : 0));  // 48-bit reads reach 2 bytes past
     size_t o = 0;
     h.off_cnt = o;
     o = align256(o + CNT_WORDS * 4);
@@ -893,6 +1015,8 @@ static void launch_digest_t(const PassArgs& a, Dispatch& st) {
 }
 template <int K>
 static void launch_digest_k(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
+    if (h.lane)
+        return;  // k_chain_lane wrote the digests from its LDS tables
     if (h.res && h.e32)
         st(&k_digest_res<K, uint32_t>, dim3((a.count + 255) / 256), dim3(256), 0, a.S, a.count, a.tables,
            a.table_bytes, a.digests);
@@ -924,9 +1048,20 @@ static void launch_chain_res_t(const PassArgs& a, Dispatch& st) {
        reinterpret_cast<const M*>(a.map), a.map_stride, a.tables, a.table_bytes, a.walks, a.rnds, a.cnt, a.fail,
        a.probes, a.item0, a.item1, a.state, a.state ? 1u : 0u);
 }
+template <int K, bool BGH, class E>
+static void launch_chain_lane_t(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
+    const uint32_t lanes = h.lp.lanes;
+    st(&k_chain_lane<K, BGH, E>, dim3((a.count + lanes - 1) / lanes), dim3(64), h.lp.lds, a.S, a.first, a.count, lanes,
+       reinterpret_cast<const uint64_t*>(a.map), a.map_stride, static_cast<const uint64_t*>(a.walks),
+       static_cast<const uint64_t*>(a.rnds), a.cnt, a.fail, a.probes, a.digests);
+}
 template <int K, bool BGH>
 static void launch_chain_kb(const HbmPlan& h, const PassArgs& a, Dispatch& st) {
-    if (h.res && h.stream64 && h.e32)
+    if (h.lane && h.lp.e16)
+        launch_chain_lane_t<K, BGH, uint16_t>(h, a, st);
+    else if (h.lane)
+        launch_chain_lane_t<K, BGH, uint32_t>(h, a, st);
+    else if (h.res && h.stream64 && h.e32)
         launch_chain_res_t<K, BGH, uint64_t, uint32_t>(a, st);
     else if (h.res && h.stream64 && h.e44)
         launch_chain_res_t<K, BGH, uint64_t, E44>(a, st);
@@ -1042,6 +1177,11 @@ static int set_lds_limits(size_t lds_block) {
     for (const void* f : {B1(2, true), B1(2, false), B1(3, true), B1(3, false), B1(4, true), B1(4, false)})
         if ((rc = opt(f))) return rc;
 #undef B1
+#define L1(K, B)                                                                   \
+    reinterpret_cast<const void*>(&k_chain_lane<K, B, uint16_t>), reinterpret_cast<const void*>(&k_chain_lane<K, B, uint32_t>)
+    for (const void* f : {L1(2, true), L1(2, false), L1(3, true), L1(3, false), L1(4, true), L1(4, false)})
+        if ((rc = opt(f))) return rc;
+#undef L1
     return for_each_lds_kernel(opt);
 }
 
@@ -1260,9 +1400,26 @@ struct HbmDecision {
 // runs as a graph; `windows_ok`: item windows are allowed (no per-trial map,
 // walk or random-word output).  Uses c.budget.
 static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tables, bool graph, bool windows_ok,
-                      HbmDecision& d) {
+                      HbmDecision& d, bool lane = false) {
     if (c.prp == CK_PRP_CACHE && !cache_possible(p, c.lds_block))
         return fail(CK_EINVAL, "a round cache needs half_bits <= 12 (this shape evaluates AES directly)");
+    if (lane) {
+        // Lane tier: the HBM tier's build and candidate stream, the tables in LDS.
+        const LanePlan L = plan_lane(p, c.lds_block, c.lds_cu);
+        if (!L.ok)
+            return fail(CK_EINVAL, "shape does not fit the lane tier (residual entries above 32 bits, or one "
+                                   "trial's table and ring above the LDS of a block)");
+        if (tables) return fail(CK_EINVAL, "the lane tier keeps no ordinal tables; use the HBM map layout for tables");
+        d.layout = kLayoutLane;
+        d.graph = graph;
+        d.window = p.pnum;
+        d.nbuf = hbm_buffers(c, p, count, kLayoutLane);
+        d.chunk = hbm_chunk(c, p, count, kLayoutLane, d.nbuf, p.pnum);
+        d.h = plan_hbm(p, d.chunk, kLayoutLane, p.pnum);
+        d.h.lp = L;
+        d.bp = plan_build(p, c.lds_block, c.prp);
+        return CK_OK;
+    }
     // Auto: descriptor entries with load + store probes where they fit (fastest
     // with tens of thousands of chains in flight, and reusable without a clear);
     // otherwise residual entries, still one random read per probe (measured
@@ -1307,7 +1464,7 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
 // HBM tier: build + chain per pass of up to `chunk` trial slots.  Two buffers
 // and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
-                   ck_stats* st) {
+                   ck_stats* st, bool lane = false) {
     c.budget = hbm_budget(c);
     // A workspace within 3 % of the budget is planned into as it is: growing it
     // would free and map hundreds of GiB again (seconds at the memory-capped
@@ -1320,7 +1477,8 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     // A workspace that does not fit (another context on the device took memory
     // since the budget was read) is planned again at half the budget.
     for (int attempt = 0;; ++attempt) {
-        rc = decide_hbm(c, p, count, out.tables != nullptr, graph_wanted, !out.map && !out.walks && !out.rnds, d);
+        rc = decide_hbm(c, p, count, out.tables != nullptr, graph_wanted, !out.map && !out.walks && !out.rnds, d,
+                        lane);
         if (rc) return rc;
         rc = ensure_ws(c, d.nbuf * d.h.buffer);
         if (rc != CK_ENOMEM || attempt == 8 || d.chunk == 1) break;
@@ -1355,7 +1513,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         a.state = h.nwin > 1 ? reinterpret_cast<uint32_t*>(base + h.off_state) : nullptr;
     }
     if (st) {
-        st->tier = CK_TIER_HBM;
+        st->tier = lane ? CK_TIER_LANE : CK_TIER_HBM;
         st->chunk = chunk;
     }
     const uint32_t passes = (count + chunk - 1) / chunk;
@@ -1624,24 +1782,25 @@ static int plan_launch(const Ctx& c, const ck_params& p, uint32_t count, ck_plan
         return CK_OK;
     }
     HbmDecision d;
-    const int rc = decide_hbm(c, p, std::max<uint32_t>(count, 1), false, c.graph_on, true, d);
+    const bool lane = choose_tier(c, p) == CK_TIER_LANE;
+    const int rc = decide_hbm(c, p, std::max<uint32_t>(count, 1), false, c.graph_on, true, d, lane);
     if (rc) return rc;
-    out.tier = CK_TIER_HBM;
-    out.layout = d.layout;
+    out.tier = lane ? CK_TIER_LANE : CK_TIER_HBM;
+    out.layout = lane ? CK_LAYOUT_RESIDUAL : d.layout;
     out.round_cache = d.bp.cached ? 1 : 0;
     out.lds_variant = CK_LDS_AUTO;
     out.trials_per_pass = d.chunk;
     out.buffers = d.nbuf;
     out.windows = d.h.nwin;
-    out.chain_waves = 4;  // k_chain blocks are 256 lanes
-    out.chain_lds_bytes = 0;
+    out.chain_waves = lane ? 1 : 4;  // k_chain_lane blocks are one wave, k_chain blocks 256 lanes
+    out.chain_lds_bytes = lane ? static_cast<uint32_t>(d.h.lp.lds) : 0;
     out.build_lds_bytes = static_cast<uint32_t>(d.bp.lds);
     out.workspace_bytes = uint64_t(d.nbuf) * d.h.buffer;
     return CK_OK;
 }
 
 static int apply_placement(Ctx& c, const ck_placement* w) {
-    if (!w || w->tier > CK_TIER_LDS || w->layout > CK_LAYOUT_RESIDUAL || w->prp > CK_PRP_DIRECT ||
+    if (!w || w->tier > CK_TIER_LANE || w->layout > CK_LAYOUT_RESIDUAL || w->prp > CK_PRP_DIRECT ||
         w->lds_variant > CK_LDS_ROUND)
         return fail(CK_EINVAL, "bad placement");
     c.tier = w->tier;
@@ -1730,7 +1889,7 @@ void ck_ctx_destroy(ck_ctx* ctx) {
 }
 
 int ck_ctx_set_tier(ck_ctx* ctx, uint32_t tier) {
-    if (!ctx || tier > CK_TIER_LDS) return fail(CK_EINVAL, "bad tier");
+    if (!ctx || tier > CK_TIER_LANE) return fail(CK_EINVAL, "bad tier");
     ctx->c.tier = tier;
     return CK_OK;
 }
@@ -1785,9 +1944,9 @@ static int launch_common(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44],
     std::vector<uint32_t> fails;
     HostOut out{&fails, probes, tables, nullptr, nullptr, nullptr, nullptr, digests};
     if (count) {
-        const uint32_t tier = choose_tier(ctx->c, *p);
+        const uint32_t tier = choose_tier(ctx->c, *p, tables != nullptr);
         rc = tier == CK_TIER_LDS ? run_lds(ctx->c, *p, rk, first, count, out, stats)
-                                 : run_hbm(ctx->c, *p, rk, first, count, out, stats);
+                                 : run_hbm(ctx->c, *p, rk, first, count, out, stats, tier == CK_TIER_LANE);
         if (rc) {
             // A failed call may have planned clears that never ran: the next
             // call starts from cleared tables.
@@ -1871,6 +2030,9 @@ int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t
     }
     if (c.prp != CK_PRP_DIRECT && plan_lds(*p, c.lds_block, c.lds_cu, c.lds_variant == CK_LDS_ROUND).ok)
         cands.push_back({CK_TIER_LDS, CK_LAYOUT_AUTO, 0});
+    // The lane tier only where 16 or more trials fit a CU: with fewer (n = 2^14
+    // fits one) a sample would run for hours.
+    if (plan_lane(*p, c.lds_block, c.lds_cu).lanes_per_cu >= 16) cands.push_back({CK_TIER_LANE, CK_LAYOUT_AUTO, 0});
     const uint32_t saved_tier = c.tier, saved_layout = c.layout, saved_chunk = c.chunk_cap;
     std::vector<uint32_t> want, got(sample);
     bool have_want = false;

@@ -126,7 +126,8 @@ const Rule kRules[] = {
      "--table-layout is auto, global or shared"},
     {[](const RunSpec& s) { return !one_of(s.shared_variant, {"auto", "round", "map"}); },
      "--shared-variant is auto, round or map"},
-    {[](const RunSpec& s) { return !one_of(s.tier, {"auto", "hbm", "lds"}); }, "--tier is auto, hbm or lds"},
+    {[](const RunSpec& s) { return !one_of(s.tier, {"auto", "hbm", "lds", "lane"}); },
+     "--tier is auto, hbm, lds or lane"},
     {[](const RunSpec& s) { return s.table_layout == "shared" && s.global_table; },
      "--table-layout shared and --global-table exclude each other"},
     {[](const RunSpec& s) { return s.table_layout == "shared" && s.prp_cache == "off"; },
@@ -140,7 +141,7 @@ std::string usage_text() {
            "                    [--seed 1] [--input-mode sequential|affine] [--workers 0] [--micro 131072]\n"
            "                    [--out cuckoo_results] [--stop-after 1048576] [--status]\n"
            "  placement:        [--scratch-mib MiB] [--prp-cache auto|on|off] [--table-layout auto|global|shared]\n"
-           "                    [--shared-variant auto|round|map] [--global-table] [--tier auto|hbm|lds]\n"
+           "                    [--shared-variant auto|round|map] [--global-table] [--tier auto|hbm|lds|lane]\n"
            "                    [--block-threads 64|128|256] [--warps-per-block N] [--tune] [--graph]\n"
            "  extension shape:  [--k 2..4] [--buckets M] [--kick N] [--stash S]\n";
 }

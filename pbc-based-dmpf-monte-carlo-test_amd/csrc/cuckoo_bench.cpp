@@ -38,7 +38,10 @@ ck_placement placement_of(const RunSpec& s, uint64_t budget_bytes) {
     else if (s.table_layout == "global" || s.global_table)
         w.tier = CK_TIER_HBM;
     else
-        w.tier = s.tier == "hbm" ? CK_TIER_HBM : s.tier == "lds" ? CK_TIER_LDS : CK_TIER_AUTO;
+        w.tier = s.tier == "hbm"    ? CK_TIER_HBM
+                 : s.tier == "lds"  ? CK_TIER_LDS
+                 : s.tier == "lane" ? CK_TIER_LANE
+                                    : CK_TIER_AUTO;
     w.layout = CK_LAYOUT_AUTO;
     w.prp = s.prp_cache == "on" ? CK_PRP_CACHE : s.prp_cache == "off" ? CK_PRP_DIRECT : CK_PRP_AUTO;
     w.lds_variant = s.shared_variant == "round" ? CK_LDS_ROUND : s.shared_variant == "map" ? CK_LDS_MAP : CK_LDS_AUTO;
@@ -58,10 +61,12 @@ const char* layout_name(uint32_t layout) {
 }
 
 // The planned placement in the reference's vocabulary where one exists
-// (shared-map / shared-trial for workgroup-LDS tables, cuckoo_bench.cu:527-563)
-// and <cached|direct>-global-<table layout> for the HBM tier.
+// (shared-map / shared-trial for workgroup-LDS tables, cuckoo_bench.cu:527-563),
+// <cached|direct>-lane-shared for lane-private LDS tables and
+// <cached|direct>-global-<table layout> for the HBM tier.
 std::string placement_name(const ck_plan& pl) {
     if (pl.tier == CK_TIER_LDS) return pl.lds_variant == CK_LDS_ROUND ? "shared-trial" : "shared-map";
+    if (pl.tier == CK_TIER_LANE) return std::string(pl.round_cache ? "cached" : "direct") + "-lane-shared";
     return std::string(pl.round_cache ? "cached" : "direct") + "-global-" + layout_name(pl.layout);
 }
 
@@ -109,7 +114,8 @@ void worker_main(Shared& sh, int index, int device) {
             if (!best || entries[i].trials_per_s > best->trials_per_s) best = &entries[i];
         if (best) {
             char buf[160];
-            std::snprintf(buf, sizeof buf, " tuned=%s/%s/pass%u@%.0f/s", best->tier == CK_TIER_LDS ? "lds" : "hbm",
+            std::snprintf(buf, sizeof buf, " tuned=%s/%s/pass%u@%.0f/s",
+                          best->tier == CK_TIER_LDS ? "lds" : best->tier == CK_TIER_LANE ? "lane" : "hbm",
                           layout_name(best->layout), best->chunk, best->trials_per_s);
             tuned = buf;
         }

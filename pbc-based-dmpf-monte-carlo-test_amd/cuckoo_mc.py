@@ -17,7 +17,7 @@ LIB_PATH = HERE / "lib" / "libcuckoo_mc.so"
 CK_OK = 0
 BGH25, DRRT18 = 1, 2
 DERIVED = 0xFFFFFFFF
-TIER_AUTO, TIER_HBM, TIER_LDS = 0, 1, 2
+TIER_AUTO, TIER_HBM, TIER_LDS, TIER_LANE = 0, 1, 2, 3
 PENDING = 1
 SCHED_BALANCE, SCHED_FIFO, SCHED_SRPT = 0, 1, 2
 LAYOUT_AUTO, LAYOUT_MAP, LAYOUT_DESC, LAYOUT_MAP_XCHG, LAYOUT_DESC_PLAIN, LAYOUT_RESIDUAL = 0, 1, 2, 3, 4, 5

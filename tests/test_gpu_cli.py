@@ -43,6 +43,7 @@ def test_cli_golden_and_workers(tmp_path, fixtures, mode):
     for name, extra in [("baseline", ["--workers", "1", "--micro", "257"]),
                         ("multi", ["--workers", "2", "--micro", "37", "--tier", "hbm"]),
                         ("lds", ["--workers", "1", "--micro", "100", "--tier", "lds"]),
+                        ("lane", ["--workers", "1", "--micro", "100", "--tier", "lane"]),
                         ("tuned", ["--workers", "1", "--micro", "256", "--tune"]),
                         ("graph", ["--workers", "2", "--micro", "100", "--graph"])]:
         d = tmp_path / name
@@ -54,6 +55,8 @@ def test_cli_golden_and_workers(tmp_path, fixtures, mode):
         assert s["status"] == "complete" and s["completed"] == 8192 and s["completed_batches"] == 32
         assert s["failed"] == len(want)
         assert failures(d) == want, name
+        if name == "lane":
+            assert s["table_layout_planned"] == "cached-lane-shared"
 
 
 @pytest.mark.parametrize("mode", ["bgh25", "drrt18"])
