@@ -380,21 +380,34 @@ struct ResCodec {
 
 // Candidate stream of new items: M = u64 is one packed descriptor per item
 // (bucket w at bits [w*bb, (w+1)*bb)), otherwise K map entries per item.
+// The chains read the next item's Word one item ahead (raw) and unpack it only
+// when that item comes into hand: unpacking at load time would wait for the
+// load there and then, a memory round trip on every item advance.
 template <int K, class M>
 struct CandStream {
+    using Word = std::conditional_t<sizeof(M) == 8, uint64_t, Cands>;
     const M* m;
     uint32_t bb;
     uint64_t bmask;
-    __device__ __forceinline__ void load(uint32_t item, Cands& c) const {
+    __device__ __forceinline__ Word raw(uint32_t item) const {
         if constexpr (sizeof(M) == 8) {
-            const uint64_t d = m[item];
+            return m[item];
+        } else {
+            Cands c = {};
+#pragma unroll
+            for (int w = 0; w < K; ++w) c[w] = m[item * K + w];
+            return c;
+        }
+    }
+    __device__ __forceinline__ void unpack(const Word& d, Cands& c) const {
+        if constexpr (sizeof(M) == 8) {
 #pragma unroll
             for (int w = 0; w < K; ++w) c[w] = static_cast<uint32_t>((d >> (w * bb)) & bmask);
         } else {
-#pragma unroll
-            for (int w = 0; w < K; ++w) c[w] = m[item * K + w];
+            c = d;
         }
     }
+    __device__ __forceinline__ void load(uint32_t item, Cands& c) const { unpack(raw(item), c); }
 };
 
 // Items [begin, end) that one residual-layout launch inserts; the candidate
@@ -413,9 +426,10 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
                                               uint32_t& stash_left, uint32_t& probes) {
     constexpr E kEmpty = static_cast<E>(~E(0));
     uint32_t item = span.begin, coll = 0, pr = 0;
-    Cands c = {}, nx = {};
+    Cands c = {};
+    typename CandStream<K, M>::Word nx = {};
     st.load(0, c);
-    if (item + 1 < span.end) st.load(1, nx);
+    if (item + 1 < span.end) nx = st.raw(1);
     bool failed = false;
     while (item < span.end) {
         if (rnd < K) rnd = splitmix(walk);
@@ -437,8 +451,8 @@ __device__ __forceinline__ bool chain_bgh_res(const Shape& S, const ResCodec<K, 
         }
         if (advance) {
             ++item;
-            c = nx;
-            if (item + 1 < span.end) st.load(item + 1 - span.begin, nx);
+            st.unpack(nx, c);
+            if (item + 1 < span.end) nx = st.raw(item + 1 - span.begin);
             coll = 0;
         } else {
             R.dec(old, b, c);
@@ -455,11 +469,11 @@ __device__ __forceinline__ bool chain_drrt_res(const Shape& S, const ResCodec<K,
     constexpr E kEmpty = static_cast<E>(~E(0));
     uint32_t pr = 0;
     bool failed = false;
-    Cands nx = {};
-    st.load(0, nx);
+    typename CandStream<K, M>::Word nx = st.raw(0);
     for (uint32_t item = span.begin; item < span.end && !failed; ++item) {
-        Cands c = nx;
-        if (item + 1 < span.end) st.load(item + 1 - span.begin, nx);
+        Cands c = {};
+        st.unpack(nx, c);
+        if (item + 1 < span.end) nx = st.raw(item + 1 - span.begin);
         bool placed = false;
 #pragma unroll
         for (int w = 0; w < K; ++w) {
