@@ -479,6 +479,45 @@ def config1_leg(ck):
             "failures_equal": list(f) == list(of)}, list(f) == list(of)
 
 
+def tiers_leg(ck):
+    """The on-chip lane tier (a trial per lane, its table in LDS) against the
+    HBM tier on small tables, where it could win: per shape one pass of
+    65,536 trials (no build/chain overlap, so each kernel runs alone; chain
+    rate = probes / chain time) and 2^20 trials in overlapped passes (end to
+    end).  Failure lists and probe totals of the two tiers must agree."""
+    shapes = [("n=256 m=384 k=3", dict(pnum=256, buckets=384, k=3)),
+              ("n=1024 m=1536 k=3", dict(pnum=1024, buckets=1536, k=3)),
+              ("n=1024 m=1331 k=2", dict(pnum=1024, buckets=1331, k=2))]
+    out, ok = [], True
+    for name, kw in shapes:
+        p = ck.params("bgh25", kw["pnum"], 16, k=kw["k"], buckets=kw["buckets"], kick=500)
+        row = {"workload": f"bgh25 {name} kick 500 ell_i 16 seed 0"}
+        res = {}
+        for tier, tname in ((ck.TIER_HBM, "hbm"), (ck.TIER_LANE, "lane")):
+            with ck.Context(0) as ctx:
+                ctx.set_tier(tier)
+                if tier == ck.TIER_LANE:
+                    row["lane_lds_per_block"] = ctx.plan(p, 1 << 16).chain_lds_bytes
+                ctx.set_chunk(1 << 16)
+                ctx.launch(p, ck.batch_key(0, 1), 0, 1 << 16)  # untimed: workspace
+                f1, st1 = ctx.launch(p, ck.batch_key(0, 0), 0, 1 << 16)
+                ctx.set_chunk(0)
+                ctx.launch(p, ck.batch_key(0, 1), 0, 1 << 20)
+                t0 = time.perf_counter()
+                f, st = ctx.launch(p, ck.batch_key(0, 0), 0, 1 << 20)
+                dt = time.perf_counter() - t0
+            res[tname] = (list(map(int, f)), st.probes)
+            row[f"{tname}_one_pass_chain_gprobes_per_s"] = st1.probes / st1.chain_ms / 1e6 if st1.chain_ms else None
+            row[f"{tname}_one_pass_build_ms"] = st1.build_ms
+            row[f"{tname}_trials_per_s"] = (1 << 20) / dt
+        row["lane_vs_hbm"] = row["lane_trials_per_s"] / row["hbm_trials_per_s"]
+        row["results_equal"] = res["hbm"] == res["lane"]
+        ok = ok and row["results_equal"]
+        out.append(row)
+        log(f"tiers: {name}: lane {row['lane_trials_per_s']:.0f} vs hbm {row['hbm_trials_per_s']:.0f} trials/s")
+    return out, ok
+
+
 def graph_leg(ck, p, steps: int, trials: int):
     """Graph mode against kernel-by-kernel launches on three launch sizes:
     the headline steps (2^20 trials), the n = 1024 point (1e5 trials per
@@ -589,6 +628,7 @@ def main():
     ap.add_argument("--no-hbm", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-mode comparison")
+    ap.add_argument("--no-tiers", action="store_true", help="skip the lane-tier comparison")
     ap.add_argument("--tune-sample", type=int, default=1 << 17, help="trials per autotuner candidate")
     ap.add_argument("--hbm-only", action="store_true", help="run only the n=2^20 leg (for counter profiles)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -740,6 +780,9 @@ def main():
     if not args.no_cpu:
         rec["config1"], good = config1_leg(ck)
         log(f"config 1: {rec['config1']['gpu_trials_per_s']:.0f} trials/s")
+        ok = ok and good
+    if not args.no_tiers:
+        rec["lane_tier"], good = tiers_leg(ck)
         ok = ok and good
     if not args.no_hbm:
         rec["hbm_tier"] = hbm_leg(ck, args.hbm_trials, layout)
