@@ -46,13 +46,29 @@ __device__ __forceinline__ void derive_uniform(const LdsTab& T, const RoundKeys&
     for (int i = 0; i < 44; ++i) ts.rk[i] = __builtin_amdgcn_readfirstlane(ts.rk[i]);
 }
 
-// The wave's round cache: cache[r<<half | right] = F_r(right) & mask.  Each
-// lane encrypts four entries at a time (entries is a multiple of 256 for
-// half >= 5), one per quarter of the cache.
+// The wave's round cache: cache[r<<half | right] = F_r(right) & mask.  For
+// half >= 8 the rounds r are taken in turn: the x-free part of AES rounds 1
+// and 2 is computed once per r (cache_round) and each lane encrypts four
+// entries of round r at a time from there (126 of the block's 148 table
+// lookups).  Smaller caches encrypt four entries at a time, one per quarter
+// of the cache (entries is a multiple of 256 for half >= 5).
 __device__ __forceinline__ void build_cache(const LdsTab& T, const TrialSeed& ts, const Shape& S, uint16_t* cache,
                                             uint32_t lane) {
     const uint32_t entries = 8u << S.half, rmask = (1u << S.half) - 1u;
-    if ((entries & 255u) == 0) {
+    if (S.half >= 8) {
+        const uint32_t per_round = 1u << S.half;
+        for (uint32_t r = 0; r < 8; ++r) {
+            const CacheRound cr = cache_round(T, ts.rk, r, S.half);
+            uint16_t* row = cache + (r << S.half);
+            for (uint32_t x0 = lane; x0 < per_round; x0 += 256) {
+                const uint32_t x[4] = {x0, x0 + 64, x0 + 128, x0 + 192};
+                uint32_t out[4];
+                aes_cache_n<4>(T, ts.rk, cr, x, out);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) row[x[i]] = static_cast<uint16_t>(out[i] & S.mask);
+            }
+        }
+    } else if ((entries & 255u) == 0) {
         const uint32_t q = entries >> 2;
         for (uint32_t e = lane; e < q; e += 64) {
             uint32_t in[4][4], out[4];

@@ -192,6 +192,74 @@ __device__ __forceinline__ void aes_last_n(const LdsTab& T, const uint32_t* rk, 
                  rk[43];
 }
 
+// Round-cache blocks share all but the last word: the block of entry
+// (r, x) is ("CESP", r, half, x) with x < 2^half <= 2^16, so after the first
+// AddRoundKey only bytes 2 and 3 of column 3 depend on x.  Round 1 then
+// depends on x in two T-table terms only (column 0 through byte 3, column 1
+// through byte 2; columns 2 and 3 are fixed), and round 2 in the eight terms
+// that read columns 0 and 1.  CacheRound holds, for one (key, r, half), the x-free
+// part of each: round-1 columns 0 and 1 without their x term, and the round-2
+// columns without their terms from columns 0 and 1 (round key included).
+struct CacheRound {
+    uint32_t c0, c1, d0, d1, d2, d3;
+    uint32_t k2, k3;  // bytes 2 and 3 of rk[3], the key bytes that x meets
+};
+__device__ __forceinline__ CacheRound cache_round(const LdsTab& T, const uint32_t* rk, uint32_t r, uint32_t half) {
+    const uint32_t s0 = 0x43455350u ^ rk[0], s1 = r ^ rk[1], s2 = half ^ rk[2], s3 = rk[3];
+    CacheRound c;
+    c.c0 = xor3(T.t(s0 >> 24), rotr(T.t((s1 >> 16) & 255), 8), rotr(T.t((s2 >> 8) & 255), 16)) ^ rk[4];
+    c.c1 = xor3(T.t(s1 >> 24), rotr(T.t((s2 >> 16) & 255), 8), rotr(T.t(s0 & 255), 24)) ^ rk[5];
+    const uint32_t n2 = xor3(xor3(T.t(s2 >> 24), rotr(T.t((s3 >> 16) & 255), 8), rotr(T.t((s0 >> 8) & 255), 16)),
+                             rotr(T.t(s1 & 255), 24), rk[6]);
+    const uint32_t n3 = xor3(xor3(T.t(s3 >> 24), rotr(T.t((s0 >> 16) & 255), 8), rotr(T.t((s1 >> 8) & 255), 16)),
+                             rotr(T.t(s2 & 255), 24), rk[7]);
+    c.d0 = xor3(rotr(T.t((n2 >> 8) & 255), 16), rotr(T.t(n3 & 255), 24), rk[8]);
+    c.d1 = xor3(rotr(T.t((n2 >> 16) & 255), 8), rotr(T.t((n3 >> 8) & 255), 16), rk[9]);
+    c.d2 = xor3(T.t(n2 >> 24), rotr(T.t((n3 >> 16) & 255), 8), rk[10]);
+    c.d3 = xor3(T.t(n3 >> 24), rotr(T.t(n2 & 255), 24), rk[11]);
+    c.k2 = (s3 >> 8) & 255;
+    c.k3 = s3 & 255;
+    return c;
+}
+
+// The last output word of N round-cache blocks (r, x[i]) sharing r: rounds 1
+// and 2 from CacheRound, then rounds 3..10 as in aes_last_n.  Equal to
+// aes_last(T, rk, "CESP", r, half, x[i]) for x[i] < 2^16.
+template <int N>
+__device__ __forceinline__ void aes_cache_n(const LdsTab& T, const uint32_t* rk, const CacheRound& c,
+                                            const uint32_t (&x)[N], uint32_t (&out)[N]) {
+    uint32_t s[N][4];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const uint32_t n0 = c.c0 ^ rotr(T.t(c.k3 ^ (x[i] & 255)), 24);
+        const uint32_t n1 = c.c1 ^ rotr(T.t(c.k2 ^ ((x[i] >> 8) & 255)), 16);
+        s[i][0] = xor3(c.d0, T.t(n0 >> 24), rotr(T.t((n1 >> 16) & 255), 8));
+        s[i][1] = xor3(c.d1, T.t(n1 >> 24), rotr(T.t(n0 & 255), 24));
+        s[i][2] = xor3(c.d2, rotr(T.t((n0 >> 8) & 255), 16), rotr(T.t(n1 & 255), 24));
+        s[i][3] = xor3(c.d3, rotr(T.t((n0 >> 16) & 255), 8), rotr(T.t((n1 >> 8) & 255), 16));
+    }
+#pragma unroll
+    for (int r = 3; r < 10; ++r) {
+        uint32_t n[N][4];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                n[i][j] = xor3(xor3(T.t(s[i][j] >> 24), rotr(T.t((s[i][(j + 1) & 3] >> 16) & 255), 8),
+                                    rotr(T.t((s[i][(j + 2) & 3] >> 8) & 255), 16)),
+                               rotr(T.t(s[i][(j + 3) & 3] & 255), 24), rk[4 * r + j]);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s[i][j] = n[i][j];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        out[i] = ((T.s(s[i][3] >> 24) << 24) | (T.s((s[i][0] >> 16) & 255) << 16) | (T.s((s[i][1] >> 8) & 255) << 8) |
+                  T.s(s[i][2] & 255)) ^
+                 rk[43];
+}
+
 // The k candidates of one element through the cached Feistel network in
 // lockstep: each pass applies the 8 rounds to every candidate (the k dependent
 // LDS chains overlap) and keeps the result only for candidates still outside
