@@ -126,6 +126,29 @@ def test_plan_describe_follows_the_placement_request():
         ck.plan_describe(g, 256, tier=ck.TIER_LDS, prp=ck.PRP_DIRECT)
 
 
+def test_plan_describe_lane_tier():
+    """The lane tier's plan (no device): one-wave blocks, W lanes each with
+    the whole table (16-bit residual entries where (k-1)*bits + 2 <= 16, else
+    32-bit) and a 64-word ring in LDS, W chosen for the most trials per CU of
+    160 KB; shapes whose entries need more than 32 bits are refused."""
+    def best(buckets, eb):
+        cands = []
+        for w in range(1, 65):
+            blk = ((w * buckets * eb + 15) // 16) * 16 + w * 64 * 8
+            if blk <= 160 << 10:
+                cands.append(((160 << 10) // blk * w, w, blk))
+        return max(cands)[2]
+    for (n, m, k, eb, blk) in [(256, 384, 3, 4, 81920), (1024, 1536, 3, 4, 159744), (1024, 1331, 2, 2, 161888)]:
+        p = ck.params("bgh25", n, 16, k=k, buckets=m, kick=500)
+        pl = ck.plan_describe(p, 1 << 16, tier=ck.TIER_LANE)
+        assert (pl.tier, pl.layout, pl.round_cache) == (ck.TIER_LANE, ck.LAYOUT_RESIDUAL, 1)
+        assert pl.chain_waves == 1 and pl.chain_lds_bytes == blk == best(m, eb)
+        assert pl.trials_per_pass == 1 << 16 and pl.windows == 1
+        assert ck.plan_describe(p, 1 << 16).tier == ck.TIER_HBM  # auto
+    with pytest.raises(ck.CkError):
+        ck.plan_describe(ck.params("bgh25", 1 << 20, 20, buckets=1363149, kick=1000, stash=2), 1024, tier=ck.TIER_LANE)
+
+
 def test_trial_inputs_match_the_oracle():
     """ck_trial_inputs_of / ck_round_cache (the trial API's host inputs) equal
     the oracle's trial setup and round cache."""
