@@ -626,7 +626,7 @@ def main():
     ap.add_argument("--hbm-trials", type=int, default=0,
                     help="trials of the n=2^20 leg (default 0: two full memory-capped passes)")
     ap.add_argument("--no-hbm", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="run the headline with the rule-based placement")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-mode comparison")
     ap.add_argument("--no-tiers", action="store_true", help="skip the lane-tier comparison")
     ap.add_argument("--tune-sample", type=int, default=1 << 17, help="trials per autotuner candidate")
@@ -661,6 +661,22 @@ def main():
     if args.chunk:
         ctx.set_chunk(args.chunk)
     ctx.set_layout(layout)
+    # With every placement knob left to the library, the context's placement
+    # (tier, table layout, trials per pass) is the autotuner's pick on a sample
+    # of batch 0, measured here before the warmup steps: which layout is
+    # fastest at this shape differs from box to box (the descriptor and the
+    # u32 residual entries have each been measured 7-9 % ahead of the other).
+    names = {0: "auto", 1: "map", 2: "desc", 3: "map-xchg", 4: "desc-plain", 5: "residual"}
+    tuned = args.layout == "auto" and args.tier == "auto" and not args.chunk and not args.no_tune
+    tune_rec = None
+    if tuned:
+        ents = ctx.tune(p, ck.batch_key(SEED, 0), args.tune_sample)
+        tune_rec = {"sample_trials": args.tune_sample, "candidates": [
+            {"tier": {1: "hbm", 2: "lds", 3: "lane"}.get(e["tier"], "auto"), "layout": names[e["layout"]],
+             "chunk": e["chunk"], "trials_per_s": round(e["trials_per_s"])} for e in ents]}
+        best = max(ents, key=lambda e: e["trials_per_s"])
+        log(f"autotune: best tier {best['tier']} layout {names[best['layout']]} chunk {best['chunk']} "
+            f"{best['trials_per_s']:.0f} trials/s")
 
     if args.copy_probe_mib:
         n = args.copy_probe_mib << 18  # float32 elements
@@ -730,6 +746,7 @@ def main():
         "config": {"workload": "bgh25 n=16384 m=24576 k=3 stash=0 kick=500 ell_i=16 seed=1 BT=20 sequential",
                    "trials_per_step": t, "tier": {1: "hbm", 2: "lds", 3: "lane"}.get(agg["tier"], "auto"),
                    "layout": args.layout, "layout_used": layout_name,
+                   "placement": "autotuned before the warmup (autotune)" if tuned else "as requested",
                    "trials_per_pass": agg["chunk"]},
         "insertions_per_s": value * p.pnum,
         "probes_per_s": agg["probes"] / elapsed,
@@ -762,21 +779,17 @@ def main():
     }
     rec["roofline"]["random_access_ceiling"] = random_access_ceiling(probes_per_launch / chain_avg_s,
                                                                      prof["requests"] if prof else None)
+    # The pass-buffer placement search of this context (chain ms of one pass in
+    # each candidate buffer, the two kept).
+    ms, kept = ctx.place_report()
+    rec["config"]["placement_search"] = {"candidate_chain_ms": [round(x, 2) for x in ms], "kept": list(kept)}
     ok = True
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)
         log(f"cpu baseline: {rec['cpu_baseline']['value']:.0f} trials/s on {rec['cpu_baseline']['cores']} cores")
     ctx.close()
-    if not args.no_tune:
-        with ck.Context(0) as tctx:
-            names = {0: "auto", 1: "map", 2: "desc", 3: "map-xchg", 4: "desc-plain", 5: "residual"}
-            ents = tctx.tune(p, ck.batch_key(SEED, 0), args.tune_sample)
-        rec["autotune"] = {"sample_trials": args.tune_sample, "candidates": [
-            {"tier": {1: "hbm", 2: "lds", 3: "lane"}.get(e["tier"], "auto"), "layout": names[e["layout"]], "chunk": e["chunk"],
-             "trials_per_s": round(e["trials_per_s"])} for e in ents]}
-        best = max(ents, key=lambda e: e["trials_per_s"])
-        log(f"autotune: best tier {best['tier']} layout {names[best['layout']]} chunk {best['chunk']} "
-              f"{best['trials_per_s']:.0f} trials/s")
+    if tune_rec:
+        rec["autotune"] = tune_rec
     if not args.no_cpu:
         rec["config1"], good = config1_leg(ck)
         log(f"config 1: {rec['config1']['gpu_trials_per_s']:.0f} trials/s")

@@ -133,7 +133,20 @@ int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
  * return the bucket map, walks or random words.  No reference counterpart (the
  * reference keeps no per-trial candidate stream). */
 int ck_ctx_set_window(ck_ctx* ctx, uint32_t items);
-/* Forces an HBM-tier table layout (CK_LAYOUT_*). */
+/* Placement search for double-buffered HBM-tier passes whose pass buffers are
+ * 1 GiB or larger (outside graph mode and ck_launch_static): the library
+ * allocates up to `candidates` pass buffers, as many as its memory budget
+ * holds (0 turns the search off; default 16), runs one pass in each and keeps
+ * the two whose chains ran fastest.  On MI355X the same pass chains in
+ * 93-95 ms in most 15-21 GB buffers and in 73-79 ms in some, repeatably per
+ * buffer.  Results are unaffected. */
+int ck_ctx_set_place_search(ck_ctx* ctx, uint32_t candidates);
+/* The last search: chain ms of each candidate in allocation order (up to cap
+ * of them; *n is their number) and the candidates kept as pass buffers 0 and
+ * 1 (-1 when the context holds no searched buffers). */
+int ck_ctx_place_report(ck_ctx* ctx, float* chain_ms, uint32_t cap, uint32_t* n, int32_t kept[2]);
+
+/* Forces an HBM-tier table layout (CK_LAYOUT_*). *//* Forces an HBM-tier table layout (CK_LAYOUT_*). */
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
 /* Graph mode (on = 1): HBM-tier ck_launch calls run as one hipGraph of all
  * their passes.  A graph is built on the first call for a shape, trial count

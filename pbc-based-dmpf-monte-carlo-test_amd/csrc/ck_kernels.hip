@@ -587,6 +587,14 @@ struct Ctx {
     hipEvent_t ev_build[2][2] = {};  // [buffer][start, end]
     hipEvent_t ev_chain[2][2] = {};
     Workspace ws;
+    // Placement search (double-buffered HBM passes of at least kPlaceMinBuffer):
+    // pass buffer 1 is its own allocation, and both pass buffers are the two
+    // fastest of place_cands candidate allocations (see place_buffers).
+    Workspace ws2;
+    size_t placed_for = 0;           // pass-buffer size ws and ws2 were placed for (0: not placed)
+    uint32_t place_cands = 16;       // candidates per search, as many as the budget holds (0: no search)
+    std::vector<float> place_ms;     // last search: chain ms of each candidate, in allocation order
+    int place_kept[2] = {-1, -1};    // candidates kept as pass buffers 0 and 1
     uint32_t tier = CK_TIER_AUTO;
     uint32_t layout = CK_LAYOUT_AUTO;
     uint32_t chunk_cap = 0;
@@ -887,7 +895,7 @@ static size_t hbm_budget(const Ctx& c) {
     if (c.budget_fixed) return c.budget_fixed;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return size_t(96) << 30;
-    const size_t own = std::max(c.ws.bytes, largest_retired(c.device));  // memory a growing workspace can use
+    const size_t own = std::max(c.ws.bytes + c.ws2.bytes, largest_retired(c.device));  // memory a new workspace can use
     return std::min(kBudgetCap, (free_b + own) / 20 * 19);
 }
 
@@ -922,26 +930,35 @@ static void forget_tables(Ctx& c) {
     c.table_tags[0] = c.table_tags[1] = {};
 }
 
-// Grows the workspace.  Graphs whose kernel nodes point into the old
-// workspace are destroyed first (they could never be replayed against the new
-// one).  CK_ENOMEM when the device cannot hold `bytes`.
+// Frees (or, after graph use, retires) the workspace and a placed second pass
+// buffer.  Graphs whose kernel nodes point into them are destroyed first (they
+// could never be replayed against a new workspace).
+static int release_ws(Ctx& c) {
+    forget_tables(c);
+    for (HbmGraph& g : c.graphs)
+        if (g.exec && g.ws == c.ws.base) g.drop();
+    for (Workspace* w : {&c.ws, &c.ws2}) {
+        if (!w->base) continue;
+        if (c.ws_graphed)
+            retire_ws(c.device, *w);
+        else
+            CKH(hipFree(w->base));
+        *w = Workspace{};
+    }
+    c.ws_graphed = false;
+    c.placed_for = 0;
+    return CK_OK;
+}
+
+// Grows the workspace (one allocation).  CK_ENOMEM when the device cannot
+// hold `bytes`.
 static int ensure_ws(Ctx& c, size_t bytes) {
     if (c.ws.bytes >= bytes) return CK_OK;
     if (c.ws_static)
         return fail(CK_EINVAL, "this launch needs a " + std::to_string(bytes >> 20) +
                                    " MiB workspace; ck_launch_static's array holds " +
                                    std::to_string(c.ws.bytes >> 20) + " MiB (one trial of this shape does not fit)");
-    forget_tables(c);
-    for (HbmGraph& g : c.graphs)
-        if (g.exec && g.ws == c.ws.base) g.drop();
-    if (c.ws.base) {
-        if (c.ws_graphed)
-            retire_ws(c.device, c.ws);
-        else
-            CKH(hipFree(c.ws.base));
-    }
-    c.ws = Workspace{};
-    c.ws_graphed = false;
+    if (int rc = release_ws(c)) return rc;
     Workspace reuse;
     if (take_retired(c.device, bytes, reuse)) {
         c.ws = reuse;
@@ -1237,7 +1254,7 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
 // that slows the other pass's chains.
 static uint64_t table_plan(const Ctx& c, const ck_params& p, const HbmPlan& h, uint32_t nbuf, bool taggable) {
     uint64_t x = reinterpret_cast<uintptr_t>(c.ws.base);
-    for (uint64_t v : {uint64_t(h.buffer), uint64_t(h.off_tables), uint64_t(h.table_bytes), uint64_t(nbuf),
+    for (uint64_t v : {uint64_t(reinterpret_cast<uintptr_t>(c.ws2.base)), uint64_t(h.buffer), uint64_t(h.off_tables), uint64_t(h.table_bytes), uint64_t(nbuf),
                        uint64_t(p.k) * make_shape(p).dbits, uint64_t(taggable)}) {
         x = (x ^ v) * 0x9E3779B97F4A7C15ull;
         x ^= x >> 29;
@@ -1461,6 +1478,105 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
     return CK_OK;
 }
 
+// Placement search.  The chain rate of a pass belongs to the pass buffer it
+// runs in: at n = 2^14 (65,536 chains, 15-21 GB per buffer) the same pass
+// chained in 93-95 ms in most buffers and in 73-79 ms in some (about a third
+// of them), the same figure every time in a given buffer (eight contexts, two
+// passes each, in one process; twelve candidates per search in later runs).
+// Which memory is fast cannot be asked for, so a double-buffered plan with
+// large pass buffers allocates up to place_cands candidate buffers, runs one
+// pass (build, then the chain alone) in each, keeps the two whose chain ran
+// fastest as its pass buffers and frees the rest: the headline rate went from
+// 683K to 812-815K trials/s.  The search reruns only when a plan needs pass
+// buffers of another size.  Results do not depend on it: every pass computes
+// the same trials in any buffer.
+static constexpr size_t kPlaceMinBuffer = size_t(1) << 30;
+static bool placeable(const Ctx& c, const HbmDecision& d) {
+    return c.place_cands >= 2 && !c.ws_static && !c.graph_on && d.nbuf == 2 && d.h.nwin == 1 &&
+           d.h.buffer >= kPlaceMinBuffer && c.budget / d.h.buffer >= 3;
+}
+static int place_buffers(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, const HbmDecision& d) {
+    const size_t need = d.h.buffer;
+    if (c.placed_for == need && c.ws.bytes >= need && c.ws2.bytes >= need) return CK_OK;
+    if (int rc = release_ws(c)) return rc;
+    const uint32_t want = static_cast<uint32_t>(std::min<size_t>(c.place_cands, c.budget / need));
+    std::vector<void*> cand;
+    for (uint32_t i = 0; i < want; ++i) {
+        void* b = nullptr;
+        const hipError_t e = hipMalloc(&b, need);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            break;
+        }
+        if (e != hipSuccess) {
+            for (void* x : cand) (void)hipFree(x);
+            return fail(CK_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        cand.push_back(b);
+    }
+    if (cand.size() < 2) {
+        for (void* x : cand) (void)hipFree(x);
+        return fail(CK_ENOMEM, "device memory: cannot allocate two " + std::to_string(need >> 20) + " MiB pass buffers");
+    }
+    const bool bgh = p.mode == CK_MODE_BGH25;
+    std::vector<float> ms(cand.size(), 0.f);
+    int rc = CK_OK;
+    for (size_t i = 0; i < cand.size() && rc == CK_OK; ++i) {
+        char* base = static_cast<char*>(cand[i]);
+        PassArgs a{};
+        a.S = make_shape(p);
+        std::memcpy(a.bk.w, rk, sizeof a.bk.w);
+        a.first = first;
+        a.count = d.chunk;
+        a.map = base + d.h.off_map;
+        a.map_stride = d.h.map_stride;
+        a.tables = base + d.h.off_tables;
+        a.table_bytes = d.h.table_bytes;
+        a.walks = reinterpret_cast<uint64_t*>(base + d.h.off_walk);
+        a.rnds = reinterpret_cast<uint64_t*>(base + d.h.off_rnd);
+        a.cnt = reinterpret_cast<uint32_t*>(base + d.h.off_cnt);
+        a.fail = reinterpret_cast<uint32_t*>(base + d.h.off_fail);
+        a.clear = 1;
+        a.tag = 0;
+        a.item0 = 0;
+        a.item1 = p.pnum;
+        Dispatch bd, cd;
+        bd.stream = c.build_stream;
+        cd.stream = c.stream;
+        launch_build(p.k, d.bp, d.h, a, bd);
+        if (bd.err != hipSuccess) rc = fail(CK_EDEVICE, std::string("build launch: ") + hipGetErrorString(bd.err));
+        if (rc == CK_OK && hipEventRecord(c.ev_build[0][1], c.build_stream) != hipSuccess) rc = fail(CK_EDEVICE, "event");
+        if (rc == CK_OK && hipStreamWaitEvent(c.stream, c.ev_build[0][1], 0) != hipSuccess) rc = fail(CK_EDEVICE, "wait");
+        if (rc) break;
+        cd(&k_zero, dim3(1), dim3(64), 0, a.cnt, CNT_WORDS);
+        if (hipEventRecord(c.ev_chain[0][0], c.stream) != hipSuccess) rc = fail(CK_EDEVICE, "event");
+        if (rc == CK_OK) launch_chain(p.k, bgh, d.h, a, cd);
+        if (rc == CK_OK && cd.err != hipSuccess)
+            rc = fail(CK_EDEVICE, std::string("chain launch: ") + hipGetErrorString(cd.err));
+        if (rc == CK_OK && hipEventRecord(c.ev_chain[0][1], c.stream) != hipSuccess) rc = fail(CK_EDEVICE, "event");
+        if (rc == CK_OK && hipEventSynchronize(c.ev_chain[0][1]) != hipSuccess) rc = fail(CK_EDEVICE, "chain pass");
+        if (rc == CK_OK && hipEventElapsedTime(&ms[i], c.ev_chain[0][0], c.ev_chain[0][1]) != hipSuccess)
+            rc = fail(CK_EDEVICE, "event time");
+    }
+    if (rc) {
+        (void)hipDeviceSynchronize();
+        for (void* x : cand) (void)hipFree(x);
+        return rc;
+    }
+    std::vector<int> order(cand.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ms[x] < ms[y]; });
+    for (size_t i = 2; i < order.size(); ++i) CKH(hipFree(cand[order[i]]));
+    c.ws = Workspace{cand[order[0]], need};
+    c.ws2 = Workspace{cand[order[1]], need};
+    c.placed_for = need;
+    c.place_ms = ms;
+    c.place_kept[0] = order[0];
+    c.place_kept[1] = order[1];
+    forget_tables(c);  // the search passes wrote the tables
+    return CK_OK;
+}
+
 // HBM tier: build + chain per pass of up to `chunk` trial slots.  Two buffers
 // and two streams let the build of pass i+1 run while pass i chains.
 static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, uint32_t count, HostOut& out,
@@ -1474,13 +1590,15 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         c.graph_on && !out.probes && !out.tables && !out.keys && !out.walks && !out.rnds && !out.map && !out.digests;
     HbmDecision d;
     int rc = CK_OK;
+    bool placed = false;
     // A workspace that does not fit (another context on the device took memory
     // since the budget was read) is planned again at half the budget.
     for (int attempt = 0;; ++attempt) {
         rc = decide_hbm(c, p, count, out.tables != nullptr, graph_wanted, !out.map && !out.walks && !out.rnds, d,
                         lane);
         if (rc) return rc;
-        rc = ensure_ws(c, d.nbuf * d.h.buffer);
+        placed = placeable(c, d);
+        rc = placed ? place_buffers(c, p, rk, first, d) : ensure_ws(c, d.nbuf * d.h.buffer);
         if (rc != CK_ENOMEM || attempt == 8 || d.chunk == 1) break;
         c.budget /= 2;
     }
@@ -1493,7 +1611,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     const bool bgh = p.mode == CK_MODE_BGH25;
     PassArgs args[2];
     for (uint32_t b = 0; b < nbuf; ++b) {
-        char* base = static_cast<char*>(c.ws.base) + b * h.buffer;
+        char* base = placed && b == 1 ? static_cast<char*>(c.ws2.base) : static_cast<char*>(c.ws.base) + b * h.buffer;
         PassArgs& a = args[b];
         a.S = make_shape(p);
         std::memcpy(a.bk.w, rk, sizeof a.bk.w);
@@ -1870,11 +1988,12 @@ void ck_ctx_destroy(ck_ctx* ctx) {
         g.drop();
         if (g.aux) (void)hipFree(g.aux);
     }
-    if (ctx->c.ws.base) {
+    for (Workspace* w : {&ctx->c.ws, &ctx->c.ws2}) {
+        if (!w->base) continue;
         if (ctx->c.ws_graphed)
-            retire_ws(ctx->c.device, ctx->c.ws);
+            retire_ws(ctx->c.device, *w);
         else
-            (void)hipFree(ctx->c.ws.base);
+            (void)hipFree(w->base);
     }
     for (auto& ev : ctx->c.ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1903,6 +2022,22 @@ int ck_ctx_set_graph(ck_ctx* ctx, uint32_t on) {
 int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials) {
     if (!ctx) return fail(CK_EINVAL, "null ctx");
     ctx->c.chunk_cap = trials;
+    return CK_OK;
+}
+
+int ck_ctx_set_place_search(ck_ctx* ctx, uint32_t candidates) {
+    if (!ctx || candidates == 1 || candidates > 64) return fail(CK_EINVAL, "candidates must be 0 or 2..64");
+    ctx->c.place_cands = candidates;
+    return CK_OK;
+}
+
+int ck_ctx_place_report(ck_ctx* ctx, float* chain_ms, uint32_t cap, uint32_t* n, int32_t kept[2]) {
+    if (!ctx || !n || !kept || (cap && !chain_ms)) return fail(CK_EINVAL, "null argument");
+    const Ctx& c = ctx->c;
+    *n = static_cast<uint32_t>(c.place_ms.size());
+    for (uint32_t i = 0; i < std::min(cap, *n); ++i) chain_ms[i] = c.place_ms[i];
+    kept[0] = c.placed_for ? c.place_kept[0] : -1;
+    kept[1] = c.placed_for ? c.place_kept[1] : -1;
     return CK_OK;
 }
 
@@ -2007,8 +2142,8 @@ int ck_launch_digest(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uin
 }
 
 // Autotuner.  Each applicable (tier, layout, trials per pass) candidate runs
-// the first sample/8 trials once (workspace growth, warm caches), then the
-// whole sample with wall-clock timing.  Every candidate must report the same
+// the first min(sample, 65536) trials once (workspace growth, placement
+// search, warm caches), then the whole sample with wall-clock timing.  Every candidate must report the same
 // failing offsets; the fastest becomes the context's setting.
 int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t sample, ck_tune_entry* entries,
                 uint32_t cap, uint32_t* n) {
@@ -2044,8 +2179,10 @@ int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t
         c.layout = k.layout;
         c.chunk_cap = k.chunk;
         uint32_t nf = 0;
-        rc = launch_common(ctx, p, rk, 0, std::max<uint32_t>(1, sample / 8), got.data(), &nf, nullptr, nullptr,
-                           nullptr, nullptr);
+        // Warm-up: a launch planned like the timed one (the same trials per
+        // pass), so that workspace growth and the placement search happen here.
+        rc = launch_common(ctx, p, rk, 0, std::min(sample, kHbmChunk), got.data(), &nf, nullptr, nullptr, nullptr,
+                           nullptr);
         const auto t0 = std::chrono::steady_clock::now();
         if (rc == CK_OK) rc = launch_common(ctx, p, rk, 0, sample, got.data(), &nf, nullptr, nullptr, nullptr, nullptr);
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -116,6 +116,8 @@ def _load():
         "ck_ctx_set_tier": (C.c_int, [vp, U32]),
         "ck_ctx_set_chunk": (C.c_int, [vp, U32]),
         "ck_ctx_set_window": (C.c_int, [vp, U32]),
+        "ck_ctx_set_place_search": (C.c_int, [vp, U32]),
+        "ck_ctx_place_report": (C.c_int, [vp, P(C.c_float), U32, P(U32), P(C.c_int32)]),
         "ck_ctx_set_layout": (C.c_int, [vp, U32]),
         "ck_ctx_set_graph": (C.c_int, [vp, U32]),
         "ck_launch_digest": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U64),
@@ -158,7 +160,8 @@ EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "
            "ck_launch_debug", "ck_launch_digest", "ck_trial_map", "ck_last_error", "ck_ctx_tune", "ck_engine_create",
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
            "ck_engine_release", "ck_engine_stats_get", "ck_trial_inputs_of", "ck_round_cache",
-           "ck_ctx_set_placement", "ck_ctx_plan", "ck_plan_describe", "ck_launch_static"]
+           "ck_ctx_set_placement", "ck_ctx_plan", "ck_plan_describe", "ck_launch_static",
+           "ck_ctx_set_place_search", "ck_ctx_place_report"]
 
 
 def _check(rc):
@@ -252,6 +255,18 @@ class Context:
 
     def set_layout(self, layout: int):
         _check(LIB.ck_ctx_set_layout(self.h, layout))
+
+    def set_place_search(self, candidates: int):
+        """Pass-buffer placement search: candidates per search (0 = off)."""
+        _check(LIB.ck_ctx_set_place_search(self.h, candidates))
+
+    def place_report(self):
+        """The last placement search: (chain ms per candidate, kept indices)."""
+        ms = (C.c_float * 64)()
+        n = C.c_uint32()
+        kept = (C.c_int32 * 2)()
+        _check(LIB.ck_ctx_place_report(self.h, ms, 64, C.byref(n), kept))
+        return [float(x) for x in ms[:min(n.value, 64)]], (kept[0], kept[1])
 
     def set_graph(self, on: bool):
         _check(LIB.ck_ctx_set_graph(self.h, 1 if on else 0))
