@@ -133,8 +133,9 @@ int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
  * return the bucket map, walks or random words.  No reference counterpart (the
  * reference keeps no per-trial candidate stream). */
 int ck_ctx_set_window(ck_ctx* ctx, uint32_t items);
-/* Placement search for double-buffered HBM-tier passes whose pass buffers are
- * 1 GiB or larger (outside graph mode and ck_launch_static): the library
+/* Placement search for double-buffered HBM-tier passes of 16,384 or more
+ * trials whose pass buffers are 1 GiB or larger (outside graph mode and
+ * ck_launch_static): the library
  * allocates up to `candidates` pass buffers, as many as its memory budget
  * holds (0 turns the search off; default 16), runs one pass in each and keeps
  * the two whose chains ran fastest.  On MI355X the same pass chains in

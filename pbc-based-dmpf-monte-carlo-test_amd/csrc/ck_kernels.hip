@@ -1490,10 +1490,14 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
 // 683K to 812-815K trials/s.  The search reruns only when a plan needs pass
 // buffers of another size.  Results do not depend on it: every pass computes
 // the same trials in any buffer.
+// Searched plans: passes of at least kPlaceMinChains chains (the random-access
+// path loaded, each candidate's pass short next to the run it serves; a pass
+// of 512 chains at n = 2^20 takes seconds) in buffers of at least 1 GiB.
 static constexpr size_t kPlaceMinBuffer = size_t(1) << 30;
+static constexpr uint32_t kPlaceMinChains = 1u << 14;
 static bool placeable(const Ctx& c, const HbmDecision& d) {
     return c.place_cands >= 2 && !c.ws_static && !c.graph_on && d.nbuf == 2 && d.h.nwin == 1 &&
-           d.h.buffer >= kPlaceMinBuffer && c.budget / d.h.buffer >= 3;
+           d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer && c.budget / d.h.buffer >= 3;
 }
 static int place_buffers(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, const HbmDecision& d) {
     const size_t need = d.h.buffer;
