@@ -627,6 +627,8 @@ def main():
                     help="trials of the n=2^20 leg (default 0: two full memory-capped passes)")
     ap.add_argument("--no-hbm", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="run the headline with the rule-based placement")
+    ap.add_argument("--place-search", type=int, default=16,
+                    help="candidate pass buffers the headline context times before keeping two (0: no search)")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-mode comparison")
     ap.add_argument("--no-tiers", action="store_true", help="skip the lane-tier comparison")
     ap.add_argument("--tune-sample", type=int, default=1 << 17, help="trials per autotuner candidate")
@@ -661,6 +663,9 @@ def main():
     if args.chunk:
         ctx.set_chunk(args.chunk)
     ctx.set_layout(layout)
+    # Pass buffers: the two fastest of up to 16 candidate allocations (the
+    # chain rate belongs to the memory a buffer lands in).
+    ctx.set_place_search(args.place_search)
     # With every placement knob left to the library, the context's placement
     # (tier, table layout, trials per pass) is the autotuner's pick on a sample
     # of batch 0, measured here before the warmup steps: which layout is

@@ -137,10 +137,12 @@ int ck_ctx_set_window(ck_ctx* ctx, uint32_t items);
  * trials whose pass buffers are 1 GiB or larger (outside graph mode and
  * ck_launch_static): the library
  * allocates up to `candidates` pass buffers, as many as its memory budget
- * holds (0 turns the search off; default 16), runs one pass in each and keeps
- * the two whose chains ran fastest.  On MI355X the same pass chains in
- * 93-95 ms in most 15-21 GB buffers and in 73-79 ms in some, repeatably per
- * buffer.  Results are unaffected. */
+ * holds (0, the default, turns the search off), runs one pass in each and
+ * keeps the two whose chains ran fastest; it searches again when a plan needs
+ * pass buffers of another size, so it suits contexts that run many passes of
+ * one shape.  On MI355X the same pass chains in 93-95 ms in most 15-21 GB
+ * buffers and in 73-79 ms in some, repeatably per buffer.  Results are
+ * unaffected. */
 int ck_ctx_set_place_search(ck_ctx* ctx, uint32_t candidates);
 /* The last search: chain ms of each candidate in allocation order (up to cap
  * of them; *n is their number) and the candidates kept as pass buffers 0 and

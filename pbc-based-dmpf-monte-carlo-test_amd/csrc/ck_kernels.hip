@@ -592,7 +592,7 @@ struct Ctx {
     // fastest of place_cands candidate allocations (see place_buffers).
     Workspace ws2;
     size_t placed_for = 0;           // pass-buffer size ws and ws2 were placed for (0: not placed)
-    uint32_t place_cands = 16;       // candidates per search, as many as the budget holds (0: no search)
+    uint32_t place_cands = 0;        // candidates per search, as many as the budget holds (0: no search)
     std::vector<float> place_ms;     // last search: chain ms of each candidate, in allocation order
     int place_kept[2] = {-1, -1};    // candidates kept as pass buffers 0 and 1
     uint32_t tier = CK_TIER_AUTO;
@@ -1488,8 +1488,10 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
 // pass (build, then the chain alone) in each, keeps the two whose chain ran
 // fastest as its pass buffers and frees the rest: the headline rate went from
 // 683K to 812-815K trials/s.  The search reruns only when a plan needs pass
-// buffers of another size.  Results do not depend on it: every pass computes
-// the same trials in any buffer.
+// buffers of another size, so it is for contexts that run many passes of one
+// shape (ck_ctx_set_place_search turns it on; the CLI and the headline bench
+// do).  Results do not depend on it: every pass computes the same trials in
+// any buffer.
 // Searched plans: passes of at least kPlaceMinChains chains (the random-access
 // path loaded, each candidate's pass short next to the run it serves; a pass
 // of 512 chains at n = 2^20 takes seconds) in buffers of at least 1 GiB.

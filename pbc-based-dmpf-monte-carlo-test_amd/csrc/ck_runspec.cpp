@@ -87,6 +87,7 @@ const Option kOptions[] = {
     u32_opt("--stash", &RunSpec::stash),
     u64_opt("--trials", &RunSpec::trials),
     text_opt("--tier", &RunSpec::tier),
+    u32_opt("--place-search", &RunSpec::place_search),
     switch_opt("--tune", &RunSpec::tune),
     switch_opt("--graph", &RunSpec::graph),
 };
@@ -128,6 +129,8 @@ const Rule kRules[] = {
      "--shared-variant is auto, round or map"},
     {[](const RunSpec& s) { return !one_of(s.tier, {"auto", "hbm", "lds", "lane"}); },
      "--tier is auto, hbm, lds or lane"},
+    {[](const RunSpec& s) { return s.place_search == 1 || s.place_search > 64; },
+     "--place-search is 0 (none) or 2..64 candidate buffers"},
     {[](const RunSpec& s) { return s.table_layout == "shared" && s.global_table; },
      "--table-layout shared and --global-table exclude each other"},
     {[](const RunSpec& s) { return s.table_layout == "shared" && s.prp_cache == "off"; },
@@ -143,6 +146,7 @@ std::string usage_text() {
            "  placement:        [--scratch-mib MiB] [--prp-cache auto|on|off] [--table-layout auto|global|shared]\n"
            "                    [--shared-variant auto|round|map] [--global-table] [--tier auto|hbm|lds|lane]\n"
            "                    [--block-threads 64|128|256] [--warps-per-block N] [--tune] [--graph]\n"
+           "                    [--place-search 16]\n"
            "  extension shape:  [--k 2..4] [--buckets M] [--kick N] [--stash S]\n";
 }
 

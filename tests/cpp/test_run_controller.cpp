@@ -531,6 +531,7 @@ void test_parse() {
     EXPECT(d.input_mode == "sequential" && d.workers == 0 && d.scratch_mib == 256 && !d.scratch_given);
     EXPECT(d.block_threads == 256 && d.warps_per_block == 0 && d.prp_cache == "auto");
     EXPECT(d.table_layout == "auto" && d.shared_variant == "auto" && d.k == 3 && d.stash == 0);
+    EXPECT(d.place_search == 16 && parse({"--place-search", "0"}).place_search == 0);
     const RunSpec x = parse({"--pnum", "0x40", "--seed", "18446744073709551615", "--scratch-mib", "2048",
                              "--status", "--global-table", "--workers", "3"});
     EXPECT(x.pnum == 64 && x.seed == ~0ull && x.scratch_mib == 2048 && x.scratch_given && x.status_only);
@@ -558,6 +559,8 @@ void test_parse() {
         {{"--table-layout", "x"}, "--table-layout"},
         {{"--shared-variant", "x"}, "--shared-variant"},
         {{"--tier", "x"}, "--tier"},
+        {{"--place-search", "1"}, "--place-search"},
+        {{"--place-search", "65"}, "--place-search"},
         {{"--table-layout", "shared", "--global-table"}, "exclude"},
         {{"--table-layout", "shared", "--prp-cache", "off"}, "round cache"},
         {{"--trials", "1099511627777"}, "--trials"},

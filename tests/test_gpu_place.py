@@ -13,6 +13,7 @@ def test_place_search_keeps_the_two_fastest_and_changes_no_result():
     p = ck.params("bgh25", 16384, 16, kick=500)  # 15 GB pass buffers: searched
     rk = ck.batch_key(1, 0)
     with ck.Context(0) as a:
+        a.set_place_search(16)
         f1, s1 = a.launch(p, rk, 0, 1 << 17)
         ms, kept = a.place_report()
         assert len(ms) >= 3 and all(x > 0 for x in ms)
@@ -20,8 +21,7 @@ def test_place_search_keeps_the_two_fastest_and_changes_no_result():
         assert (ms[kept[0]], ms[kept[1]]) == (ms[order[0]], ms[order[1]])
         f1b, s1b = a.launch(p, rk, 0, 1 << 17)  # same buffers, no new search
         assert a.place_report() == (ms, kept)
-    with ck.Context(0) as b:
-        b.set_place_search(0)
+    with ck.Context(0) as b:  # the default: no search
         f2, s2 = b.launch(p, rk, 0, 1 << 17)
         assert b.place_report()[1] == (-1, -1)
     assert list(f1) == list(f2) == list(f1b)
@@ -31,10 +31,12 @@ def test_place_search_keeps_the_two_fastest_and_changes_no_result():
 def test_place_search_skips_small_and_graph_plans():
     small = ck.params("bgh25", 1024, 16, kick=500)  # 0.4 GB pass buffers
     with ck.Context(0) as a:
+        a.set_place_search(16)
         a.launch(small, ck.batch_key(0, 0), 0, 1 << 17)
         assert a.place_report() == ([], (-1, -1))
     head = ck.params("bgh25", 16384, 16, kick=500)
     with ck.Context(0) as g:
+        g.set_place_search(16)
         g.set_graph(True)
         g.launch(head, ck.batch_key(1, 0), 0, 1 << 17)
         assert g.place_report()[1] == (-1, -1)
