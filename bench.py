@@ -57,13 +57,16 @@ ALGO_BUCKET_INIT_BYTES = 4
 # stream), and the profile-derived HBM bytes and DRAM requests
 # (TCC_EA0_RDREQ + TCC_EA0_WRREQ) per probe from rocprofv3 --pmc.  The chain's
 # reads are 64-byte requests, which FETCH_SIZE counts exactly.
+# The figures are those of fast pass buffers (the ones the placement search
+# keeps): in slow buffers a probe reads 65.3 / 64.5 B and makes 1.876 / 1.864
+# requests (residual / descriptor).
 CHAIN_PROFILES = {
     "desc-plain": dict(kernel="k_chain<3, bgh, u64 descriptor stream, u64 entries>", slot_bytes=16, item_bytes=8,
-                       read=64.40, write=27.48, requests=1.0086 + 0.8584,
-                       profile="profiles/k_chain_hbm_n16384_desc_traffic.md"),
+                       read=62.46, write=27.44, requests=0.9758 + 0.8570,
+                       profile="profiles/k_chain_hbm_n16384_placement_search.md"),
     "residual": dict(kernel="k_chain_res<3, bgh, u64 candidate stream, u32 entries>", slot_bytes=8, item_bytes=8,
-                     read=65.22, write=27.45, requests=1.0185 + 0.8571,
-                     profile="profiles/k_chain_res_hbm_n16384_u32_traffic.md"),
+                     read=62.69, write=27.39, requests=0.9809 + 0.8550,
+                     profile="profiles/k_chain_hbm_n16384_placement_search.md"),
 }
 # The n = 2^20 chain (44-bit residual entries, windowed): HBM bytes and DRAM
 # requests per probe and the L2 hit rate, from its counter profile.
@@ -200,23 +203,30 @@ def random_access_ceiling(probes_per_s: float, requests_per_probe):
     exe = PKG / "lib" / "ck_randbench"
     if not exe.exists():
         return None
-    out = subprocess.run([str(exe), "196608", "16384", "32768", "65536", "131072", "262144"],
+    out = subprocess.run([str(exe), "--search", "16", "196608", "16384", "32768", "65536", "131072", "262144"],
                          capture_output=True, text=True, timeout=300)
     rows = re.findall(r"^(read_write|read_only) lanes=(\d+) .*steps_per_s=([0-9.e+]+)", out.stdout, re.M)
+    found = re.findall(r"^search_(read_write|read_only) buffer=(\d+) .*steps_per_s=([0-9.e+]+)", out.stdout, re.M)
     if out.returncode != 0 or not rows:
         return {"error": out.stderr[-300:]}
     sweep = {}
     for form, lanes, rate in rows:
         sweep.setdefault(int(lanes), {})[form] = float(rate)
+    search = {}
+    for form, buf, rate in found:
+        search.setdefault(int(buf), {})[form] = float(rate)
     best_lanes = max(sweep, key=lambda n: sweep[n].get("read_write", 0))
-    rw = sweep[best_lanes]["read_write"]
-    ro = max(v.get("read_only", 0) for v in sweep.values())
+    # The best over the lane sweep and the 65,536-lane buffers of the search:
+    # the chain's pass buffers are the fastest of its own search.
+    rw = max([sweep[best_lanes]["read_write"]] + [v.get("read_write", 0) for v in search.values()])
+    ro = max([v.get("read_only", 0) for v in sweep.values()] + [v.get("read_only", 0) for v in search.values()])
     chain_req = probes_per_s * requests_per_probe if requests_per_probe else None
     ceiling_req = ro * RB_READ_ONLY_REQ_PER_STEP
     return {"pattern": "lanes x 192 KB tables, dependent random 8-byte read (+ write) per step "
                        "(load -> xor -> mul_hi -> address)",
             "read_write_steps_per_s": rw, "best_lanes": best_lanes, "read_only_steps_per_s": ro,
             "sweep": [{"lanes": n, **sweep[n]} for n in sorted(sweep)],
+            "search_65536_lanes": [{"buffer": i, **search[i]} for i in sorted(search)],
             "chain_probes_per_s": probes_per_s,
             "chain_over_read_write_steps": probes_per_s / rw,
             # The bound is the DRAM random-request rate: the chain's requests per

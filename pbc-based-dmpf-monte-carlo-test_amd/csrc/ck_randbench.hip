@@ -12,18 +12,25 @@
 // 24,576 slots would cycle within ~200 steps).  Tables are 256-byte aligned
 // and disjoint, as the chain's are.
 //
-//   ck_randbench [table_bytes] [steps] [lanes ...]
+//   ck_randbench [--search K] [table_bytes] [steps] [lanes ...]
 //
 // For each lane count it prints one line per form:
 //   read_write lanes=L table_bytes=B steps=S ms=T steps_per_s=R
 //   read_only  ...
-// The best read_write rate over the lane counts is the ceiling that bench.py
-// sets the chain's probes/s against.
+// The rate belongs to the memory the tables land in (the chain's passes run
+// 15 % faster in some allocations than in others), so with --search K it then
+// allocates K buffers for 65,536 lanes, all held at once, runs both forms in
+// each and prints
+//   search_read_write buffer=i lanes=65536 ... steps_per_s=R
+//   search_read_only  ...
+// bench.py takes the best read_only rate over all lines as the ceiling that it
+// sets the chain's DRAM request rate against.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 namespace {
@@ -73,17 +80,54 @@ __global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64
 
 }  // namespace
 
+// Both forms at `lanes` lanes over buf; prints one line per form with `tag`.
+static int measure(const char* tag, int buffer, uint64_t* buf, uint64_t pitch, uint32_t slots, uint32_t lanes,
+                   uint64_t table, uint32_t steps, unsigned long long* sink, hipEvent_t a, hipEvent_t b) {
+    const dim3 grid((lanes + 255) / 256), block(256);
+    for (int form = 0; form < 2; ++form) {
+        float ms = 0;
+        for (int rep = 0; rep < 2; ++rep) {  // the first run warms the TLB and clocks
+            CHECK(hipEventRecord(a));
+            if (form == 0)
+                hipLaunchKernelGGL(k_walk<true>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+            else
+                hipLaunchKernelGGL(k_walk<false>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+            CHECK(hipGetLastError());
+            CHECK(hipEventRecord(b));
+            CHECK(hipEventSynchronize(b));
+            CHECK(hipEventElapsedTime(&ms, a, b));
+        }
+        const double rate = double(lanes) * steps / (ms / 1e3);
+        if (buffer >= 0)
+            std::printf("%s%s buffer=%d lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n", tag,
+                        form == 0 ? "read_write" : "read_only", buffer, lanes, (unsigned long long)table, steps, ms,
+                        rate);
+        else
+            std::printf("%s%s lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n", tag,
+                        form == 0 ? "read_write" : "read_only", lanes, (unsigned long long)table, steps, ms, rate);
+        std::fflush(stdout);
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
-    const uint64_t table = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 196608ull;
-    const uint32_t steps = argc > 2 ? uint32_t(std::strtoul(argv[2], nullptr, 10)) : 16384u;
+    int arg = 1;
+    uint32_t search = 0;
+    if (argc > 2 && std::string(argv[1]) == "--search") {
+        search = uint32_t(std::strtoul(argv[2], nullptr, 10));
+        arg = 3;
+    }
+    const uint64_t table = argc > arg ? std::strtoull(argv[arg], nullptr, 10) : 196608ull;
+    const uint32_t steps = argc > arg + 1 ? uint32_t(std::strtoul(argv[arg + 1], nullptr, 10)) : 16384u;
     std::vector<uint32_t> lane_counts;
-    for (int a = 3; a < argc; ++a) lane_counts.push_back(uint32_t(std::strtoul(argv[a], nullptr, 10)));
+    for (int a = arg + 2; a < argc; ++a) lane_counts.push_back(uint32_t(std::strtoul(argv[a], nullptr, 10)));
     if (lane_counts.empty()) lane_counts = {32768u, 65536u, 131072u, 262144u};
     uint32_t max_lanes = 0;
     for (uint32_t l : lane_counts) max_lanes = l > max_lanes ? l : max_lanes;
     if (table < 8 || (table & 7) || table / 8 > 0xffffffffull || steps == 0 || max_lanes == 0 ||
-        max_lanes > (1u << 22)) {
-        std::fprintf(stderr, "usage: ck_randbench [table_bytes, multiple of 8] [steps] [lanes <= 2^22 ...]\n");
+        max_lanes > (1u << 22) || search > 64) {
+        std::fprintf(stderr,
+                     "usage: ck_randbench [--search K <= 64] [table_bytes, multiple of 8] [steps] [lanes <= 2^22 ...]\n");
         return 2;
     }
     const uint32_t slots = static_cast<uint32_t>(table / 8);
@@ -106,28 +150,30 @@ int main(int argc, char** argv) {
     hipEvent_t a, b;
     CHECK(hipEventCreate(&a));
     CHECK(hipEventCreate(&b));
-    for (uint32_t lanes : lane_counts) {
-        const dim3 grid((lanes + 255) / 256), block(256);
-        for (int form = 0; form < 2; ++form) {
-            float ms = 0;
-            for (int rep = 0; rep < 2; ++rep) {  // the first run warms the TLB and clocks
-                CHECK(hipEventRecord(a));
-                if (form == 0)
-                    hipLaunchKernelGGL(k_walk<true>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
-                else
-                    hipLaunchKernelGGL(k_walk<false>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
-                CHECK(hipGetLastError());
-                CHECK(hipEventRecord(b));
-                CHECK(hipEventSynchronize(b));
-                CHECK(hipEventElapsedTime(&ms, a, b));
-            }
-            const double rate = double(lanes) * steps / (ms / 1e3);
-            std::printf("%s lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n",
-                        form == 0 ? "read_write" : "read_only", lanes, (unsigned long long)table, steps, ms, rate);
-            std::fflush(stdout);
-        }
-    }
+    for (uint32_t lanes : lane_counts)
+        if (measure("", -1, buf, pitch, slots, lanes, table, steps, sink, a, b)) return 1;
     CHECK(hipFree(buf));
+    if (search) {
+        // Candidate buffers for 65,536 lanes, all held at once (as the library's
+        // pass-buffer search holds its candidates), as many as fit 90 % of the
+        // free memory.
+        const uint32_t lanes = 65536;
+        const uint64_t bytes = pitch * lanes * 8;
+        CHECK(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t fit = (free_b / 10 * 9) / bytes;
+        std::vector<uint64_t*> cand;
+        for (uint32_t i = 0; i < search && i < fit; ++i) {
+            uint64_t* c = nullptr;
+            CHECK(hipMalloc(&c, bytes));
+            cand.push_back(c);
+        }
+        for (size_t i = 0; i < cand.size(); ++i) {
+            hipLaunchKernelGGL(k_fill, dim3(8192), dim3(256), 0, 0, cand[i], pitch * lanes, 0x243F6A8885A308D3ull);
+            CHECK(hipGetLastError());
+            if (measure("search_", int(i), cand[i], pitch, slots, lanes, table, steps, sink, a, b)) return 1;
+        }
+        for (uint64_t* c : cand) CHECK(hipFree(c));
+    }
     CHECK(hipFree(sink));
     return 0;
 }
