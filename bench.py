@@ -637,8 +637,8 @@ def main():
                     help="trials of the n=2^20 leg (default 0: two full memory-capped passes)")
     ap.add_argument("--no-hbm", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="run the headline with the rule-based placement")
-    ap.add_argument("--place-search", type=int, default=16,
-                    help="candidate pass buffers the headline context times before keeping two (0: no search)")
+    ap.add_argument("--place-search", type=int, default=32,
+                    help="candidate table regions the headline context times before keeping two (0: no search)")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-mode comparison")
     ap.add_argument("--no-tiers", action="store_true", help="skip the lane-tier comparison")
     ap.add_argument("--tune-sample", type=int, default=1 << 17, help="trials per autotuner candidate")
@@ -797,7 +797,7 @@ def main():
     # The pass-buffer placement search of this context (chain ms of one pass in
     # each candidate buffer, the two kept).
     ms, kept = ctx.place_report()
-    rec["config"]["placement_search"] = {"candidate_chain_ms": [round(x, 2) for x in ms], "kept": list(kept)}
+    rec["config"]["placement_search"] = {"candidate_walk_ms": [round(x, 2) for x in ms], "kept": list(kept)}
     ok = True
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)

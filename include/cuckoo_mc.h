@@ -135,21 +135,23 @@ int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
 int ck_ctx_set_window(ck_ctx* ctx, uint32_t items);
 /* Placement search for double-buffered HBM-tier passes of 16,384 or more
  * trials whose pass buffers are 1 GiB or larger (outside graph mode and
- * ck_launch_static): the library
- * allocates up to `candidates` pass buffers, as many as its memory budget
- * holds (0, the default, turns the search off), runs one pass in each and
- * keeps the two whose chains ran fastest; it searches again when a plan needs
- * pass buffers of another size, so it suits contexts that run many passes of
- * one shape.  On MI355X the same pass chains in 93-95 ms in most 15-21 GB
- * buffers and in 73-79 ms in some, repeatably per buffer.  Results are
- * unaffected. */
+ * ck_launch_static).  The cuckoo tables of the two pass buffers are then their
+ * own allocations: the library allocates up to `candidates` table regions, as
+ * many as its memory budget holds (0, the default, turns the search off),
+ * times the chains' access pattern (a dependent random read and write per
+ * trial slot, each in its own table) over each, and keeps the two fastest; it
+ * searches again when a plan needs table regions of another size, so it suits
+ * contexts that run many passes of one shape.  On MI355X the random-access
+ * rate of an allocation is bimodal and repeatable per allocation: the same
+ * pass chains in 93-95 ms over most 6-13 GB table regions and in 77-79 ms
+ * over some.  Results are unaffected. */
 int ck_ctx_set_place_search(ck_ctx* ctx, uint32_t candidates);
-/* The last search: chain ms of each candidate in allocation order (up to cap
- * of them; *n is their number) and the candidates kept as pass buffers 0 and
- * 1 (-1 when the context holds no searched buffers). */
-int ck_ctx_place_report(ck_ctx* ctx, float* chain_ms, uint32_t cap, uint32_t* n, int32_t kept[2]);
+/* The last search: the walk's ms over each candidate in allocation order (up
+ * to cap of them; *n is their number) and the candidates kept as the tables
+ * of pass buffers 0 and 1 (-1 when the context holds no searched tables). */
+int ck_ctx_place_report(ck_ctx* ctx, float* walk_ms, uint32_t cap, uint32_t* n, int32_t kept[2]);
 
-/* Forces an HBM-tier table layout (CK_LAYOUT_*). *//* Forces an HBM-tier table layout (CK_LAYOUT_*). */
+/* Forces an HBM-tier table layout (CK_LAYOUT_*). */
 int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
 /* Graph mode (on = 1): HBM-tier ck_launch calls run as one hipGraph of all
  * their passes.  A graph is built on the first call for a shape, trial count

@@ -331,6 +331,30 @@ __global__ __launch_bounds__(64) void k_chain_lane(Shape S, uint32_t first, uint
     add_wave_total(pr, reinterpret_cast<unsigned long long*>(counters + CNT_PROBES));
 }
 
+// Placement search: the insertion chains' access pattern without their logic
+// -- each of `lanes` lanes walks its own table of `slots` 8-byte words (pitch
+// words apart), reading a word and writing it back changed, the next index
+// taken from the value read.  The contents are whatever the memory holds
+// (every index is below `slots`); `sink` only keeps the loads live.
+__global__ __launch_bounds__(256) void k_place_walk(uint64_t* __restrict__ base, uint64_t pitch, uint32_t slots,
+                                                    uint32_t lanes, uint32_t steps,
+                                                    unsigned long long* __restrict__ sink) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    uint64_t* t = base + uint64_t(lane) * pitch;
+    uint32_t ctr = lane * 0x9E3779B9u;
+    uint32_t i = __umulhi(lane * 0x85EBCA6Bu + 0x27D4EB2Fu, slots);
+    uint64_t acc = 0;
+    for (uint32_t s = 0; s < steps; ++s) {
+        const uint64_t v = t[i];
+        t[i] = v ^ ctr;
+        acc += v;
+        i = __umulhi(static_cast<uint32_t>(v) ^ ctr, slots);
+        ctr += 0x9E3779B9u;
+    }
+    if (acc == 0x5eedull) sink[0] = acc;
+}
+
 // LDS tier: a persistent wave per trial at a time.  Wave region layout:
 // [round cache 8<<half u16][map pnum*K M, unless ROUND][table buckets E].
 // The chain is the trial API's insert_core_k (include/cuckoo_trial.hpp) on
@@ -587,13 +611,13 @@ struct Ctx {
     hipEvent_t ev_build[2][2] = {};  // [buffer][start, end]
     hipEvent_t ev_chain[2][2] = {};
     Workspace ws;
-    // Placement search (double-buffered HBM passes of at least kPlaceMinBuffer):
-    // pass buffer 1 is its own allocation, and both pass buffers are the two
-    // fastest of place_cands candidate allocations (see place_buffers).
-    Workspace ws2;
-    size_t placed_for = 0;           // pass-buffer size ws and ws2 were placed for (0: not placed)
+    // Placement search (see place_buffers): the table regions of the two pass
+    // buffers are their own allocations, the two fastest of place_cands
+    // candidates; ws then holds the rest of both pass buffers.
+    Workspace tabs[2];
+    size_t placed_for = 0;           // table-region size tabs were placed for (0: not placed)
     uint32_t place_cands = 0;        // candidates per search, as many as the budget holds (0: no search)
-    std::vector<float> place_ms;     // last search: chain ms of each candidate, in allocation order
+    std::vector<float> place_ms;     // last search: walk ms of each candidate, in allocation order
     int place_kept[2] = {-1, -1};    // candidates kept as pass buffers 0 and 1
     uint32_t tier = CK_TIER_AUTO;
     uint32_t layout = CK_LAYOUT_AUTO;
@@ -767,8 +791,8 @@ static uint32_t choose_tier(const Ctx& c, const ck_params&, bool = false) {
 }
 
 // HBM-tier pass buffer: [counters | map C x map_bytes (x2 with windows) |
-// tables C x table_bytes | walk C | rnd C | probes C | fail C | keys C x 44 |
-// digests C | state C], twice (double-buffered).  In the descriptor layout the
+// walk C | rnd C | probes C | fail C | keys C x 44 | digests C | state C |
+// tables C x table_bytes], twice (double-buffered).  In the descriptor layout the
 // "map" is one u64 candidate descriptor per ordinal and the table entries are
 // u64.  With item windows (residual layout) the map holds the candidates of
 // `window` items, in two halves: the build of window j+1 fills one while the
@@ -816,8 +840,6 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uin
     o = align256(o + CNT_WORDS * 4);
     h.off_map = o;
     o = align256(o + h.map_bytes * chunk * (h.nwin > 1 ? 2 : 1));
-    h.off_tables = o;
-    o = align256(o + h.table_bytes * chunk);
     h.off_walk = o;
     o = align256(o + size_t(chunk) * 8);
     h.off_rnd = o;
@@ -832,6 +854,8 @@ static HbmPlan plan_hbm(const ck_params& p, uint32_t chunk, uint32_t layout, uin
     o = align256(o + size_t(chunk) * 8);
     h.off_state = o;
     o = align256(o + (h.nwin > 1 ? size_t(chunk) * 4 : 0));
+    h.off_tables = o;  // last, so that a searched placement can hold the tables apart
+    o = align256(o + h.table_bytes * chunk);
     h.buffer = o;
     return h;
 }
@@ -895,7 +919,7 @@ static size_t hbm_budget(const Ctx& c) {
     if (c.budget_fixed) return c.budget_fixed;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return size_t(96) << 30;
-    const size_t own = std::max(c.ws.bytes + c.ws2.bytes, largest_retired(c.device));  // memory a new workspace can use
+    const size_t own = std::max(c.ws.bytes + c.tabs[0].bytes + c.tabs[1].bytes, largest_retired(c.device));  // memory a new workspace can use
     return std::min(kBudgetCap, (free_b + own) / 20 * 19);
 }
 
@@ -937,7 +961,7 @@ static int release_ws(Ctx& c) {
     forget_tables(c);
     for (HbmGraph& g : c.graphs)
         if (g.exec && g.ws == c.ws.base) g.drop();
-    for (Workspace* w : {&c.ws, &c.ws2}) {
+    for (Workspace* w : {&c.ws, &c.tabs[0], &c.tabs[1]}) {
         if (!w->base) continue;
         if (c.ws_graphed)
             retire_ws(c.device, *w);
@@ -1254,7 +1278,8 @@ static int copy_rows(uint32_t* dst, const char* src, size_t pitch, size_t width,
 // that slows the other pass's chains.
 static uint64_t table_plan(const Ctx& c, const ck_params& p, const HbmPlan& h, uint32_t nbuf, bool taggable) {
     uint64_t x = reinterpret_cast<uintptr_t>(c.ws.base);
-    for (uint64_t v : {uint64_t(reinterpret_cast<uintptr_t>(c.ws2.base)), uint64_t(h.buffer), uint64_t(h.off_tables), uint64_t(h.table_bytes), uint64_t(nbuf),
+    for (uint64_t v : {uint64_t(reinterpret_cast<uintptr_t>(c.tabs[0].base)),
+                       uint64_t(reinterpret_cast<uintptr_t>(c.tabs[1].base)), uint64_t(h.buffer), uint64_t(h.off_tables), uint64_t(h.table_bytes), uint64_t(nbuf),
                        uint64_t(p.k) * make_shape(p).dbits, uint64_t(taggable)}) {
         x = (x ^ v) * 0x9E3779B97F4A7C15ull;
         x ^= x >> 29;
@@ -1478,38 +1503,52 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
     return CK_OK;
 }
 
-// Placement search.  The chain rate of a pass belongs to the pass buffer it
-// runs in: at n = 2^14 (65,536 chains, 15-21 GB per buffer) the same pass
-// chained in 93-95 ms in most buffers and in 73-79 ms in some (about a third
-// of them), the same figure every time in a given buffer (eight contexts, two
-// passes each, in one process; twelve candidates per search in later runs).
-// Which memory is fast cannot be asked for, so a double-buffered plan with
-// large pass buffers allocates up to place_cands candidate buffers, runs one
-// pass (build, then the chain alone) in each, keeps the two whose chain ran
-// fastest as its pass buffers and frees the rest: the headline rate went from
-// 683K to 812-815K trials/s.  The search reruns only when a plan needs pass
-// buffers of another size, so it is for contexts that run many passes of one
-// shape (ck_ctx_set_place_search turns it on; the CLI and the headline bench
-// do).  Results do not depend on it: every pass computes the same trials in
-// any buffer.
-// Searched plans: passes of at least kPlaceMinChains chains (the random-access
-// path loaded, each candidate's pass short next to the run it serves; a pass
-// of 512 chains at n = 2^20 takes seconds) in buffers of at least 1 GiB.
+// Placement search.  The chain rate of a pass belongs to the memory its
+// tables land in: at n = 2^14 (65,536 chains) the same pass chained in 93-95
+// ms in most pass buffers and in 73-79 ms in some, the same figure every time
+// in a given buffer, and a dependent random read-write walk shows the same two
+// rates (20.5 and 24.6 G steps/s) per allocation, from 6 GiB to 64 GiB, with
+// fast allocations rarer the larger they are.  Which memory is fast cannot be
+// asked for (neither the contiguous-allocation flag nor the virtual-memory API
+// changes the odds), so a double-buffered plan with large pass buffers gives
+// each pass buffer's tables an allocation of their own: it allocates up to
+// place_cands candidate table regions, times k_place_walk over each (the
+// chains' access pattern, a few ms) and keeps the two fastest.  The rest of
+// both pass buffers (candidate maps, per-slot words) shares one allocation;
+// with fast tables the chains ran at the fast rate wherever it landed.  The
+// search reruns only when a plan needs tables of another size, so it is for
+// contexts that run many passes of one shape (ck_ctx_set_place_search turns
+// it on; the CLI and the headline bench do).  Results do not depend on it:
+// every pass computes the same trials in any buffer.
+// Searched plans: passes of at least kPlaceMinChains chains (a walk over
+// fewer lanes does not load the random-access path) in buffers of at least 1 GiB.
 static constexpr size_t kPlaceMinBuffer = size_t(1) << 30;
+static constexpr uint32_t kPlaceWalkSteps = 4096;
 static constexpr uint32_t kPlaceMinChains = 1u << 14;
 static bool placeable(const Ctx& c, const HbmDecision& d) {
     return c.place_cands >= 2 && !c.ws_static && !c.graph_on && d.nbuf == 2 && d.h.nwin == 1 &&
-           d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer && c.budget / d.h.buffer >= 3;
+           d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer &&
+           c.budget >= 2 * d.h.off_tables + 3 * (d.h.buffer - d.h.off_tables);
 }
-static int place_buffers(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t first, const HbmDecision& d) {
-    const size_t need = d.h.buffer;
-    if (c.placed_for == need && c.ws.bytes >= need && c.ws2.bytes >= need) return CK_OK;
+static int place_buffers(Ctx& c, const HbmDecision& d) {
+    const size_t rest = d.h.off_tables, tabs = d.h.buffer - d.h.off_tables;
+    if (c.placed_for == tabs && c.ws.bytes >= 2 * rest && c.tabs[0].bytes >= tabs && c.tabs[1].bytes >= tabs)
+        return CK_OK;
     if (int rc = release_ws(c)) return rc;
-    const uint32_t want = static_cast<uint32_t>(std::min<size_t>(c.place_cands, c.budget / need));
+    hipError_t e = hipMalloc(&c.ws.base, 2 * rest);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        c.ws.base = nullptr;
+        return fail(CK_ENOMEM, "device memory: cannot allocate the pass buffers");
+    }
+    if (e != hipSuccess) return fail(CK_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c.ws.bytes = 2 * rest;
+    const size_t room = c.budget > 2 * rest ? c.budget - 2 * rest : 0;
+    const uint32_t want = static_cast<uint32_t>(std::min<size_t>(c.place_cands, room / tabs));
     std::vector<void*> cand;
     for (uint32_t i = 0; i < want; ++i) {
         void* b = nullptr;
-        const hipError_t e = hipMalloc(&b, need);
+        e = hipMalloc(&b, tabs);
         if (e == hipErrorOutOfMemory) {
             (void)hipGetLastError();
             break;
@@ -1522,64 +1561,37 @@ static int place_buffers(Ctx& c, const ck_params& p, const uint32_t rk[44], uint
     }
     if (cand.size() < 2) {
         for (void* x : cand) (void)hipFree(x);
-        return fail(CK_ENOMEM, "device memory: cannot allocate two " + std::to_string(need >> 20) + " MiB pass buffers");
+        return fail(CK_ENOMEM, "device memory: cannot allocate two " + std::to_string(tabs >> 20) + " MiB table regions");
     }
-    const bool bgh = p.mode == CK_MODE_BGH25;
+    // Each candidate: the chains' access pattern over its table regions (one
+    // lane per trial slot, a dependent random read and write of 8-byte words
+    // of the lane's own table), once to warm the translations, once timed.
+    const uint64_t pitch = d.h.table_bytes / 8;
+    const uint32_t lanes = d.chunk, steps = kPlaceWalkSteps;
+    auto* sink = reinterpret_cast<unsigned long long*>(c.ws.base);  // the first pass's counters, cleared per pass
     std::vector<float> ms(cand.size(), 0.f);
-    int rc = CK_OK;
-    for (size_t i = 0; i < cand.size() && rc == CK_OK; ++i) {
-        char* base = static_cast<char*>(cand[i]);
-        PassArgs a{};
-        a.S = make_shape(p);
-        std::memcpy(a.bk.w, rk, sizeof a.bk.w);
-        a.first = first;
-        a.count = d.chunk;
-        a.map = base + d.h.off_map;
-        a.map_stride = d.h.map_stride;
-        a.tables = base + d.h.off_tables;
-        a.table_bytes = d.h.table_bytes;
-        a.walks = reinterpret_cast<uint64_t*>(base + d.h.off_walk);
-        a.rnds = reinterpret_cast<uint64_t*>(base + d.h.off_rnd);
-        a.cnt = reinterpret_cast<uint32_t*>(base + d.h.off_cnt);
-        a.fail = reinterpret_cast<uint32_t*>(base + d.h.off_fail);
-        a.clear = 1;
-        a.tag = 0;
-        a.item0 = 0;
-        a.item1 = p.pnum;
-        Dispatch bd, cd;
-        bd.stream = c.build_stream;
-        cd.stream = c.stream;
-        launch_build(p.k, d.bp, d.h, a, bd);
-        if (bd.err != hipSuccess) rc = fail(CK_EDEVICE, std::string("build launch: ") + hipGetErrorString(bd.err));
-        if (rc == CK_OK && hipEventRecord(c.ev_build[0][1], c.build_stream) != hipSuccess) rc = fail(CK_EDEVICE, "event");
-        if (rc == CK_OK && hipStreamWaitEvent(c.stream, c.ev_build[0][1], 0) != hipSuccess) rc = fail(CK_EDEVICE, "wait");
-        if (rc) break;
-        cd(&k_zero, dim3(1), dim3(64), 0, a.cnt, CNT_WORDS);
-        if (hipEventRecord(c.ev_chain[0][0], c.stream) != hipSuccess) rc = fail(CK_EDEVICE, "event");
-        if (rc == CK_OK) launch_chain(p.k, bgh, d.h, a, cd);
-        if (rc == CK_OK && cd.err != hipSuccess)
-            rc = fail(CK_EDEVICE, std::string("chain launch: ") + hipGetErrorString(cd.err));
-        if (rc == CK_OK && hipEventRecord(c.ev_chain[0][1], c.stream) != hipSuccess) rc = fail(CK_EDEVICE, "event");
-        if (rc == CK_OK && hipEventSynchronize(c.ev_chain[0][1]) != hipSuccess) rc = fail(CK_EDEVICE, "chain pass");
-        if (rc == CK_OK && hipEventElapsedTime(&ms[i], c.ev_chain[0][0], c.ev_chain[0][1]) != hipSuccess)
-            rc = fail(CK_EDEVICE, "event time");
-    }
-    if (rc) {
-        (void)hipDeviceSynchronize();
-        for (void* x : cand) (void)hipFree(x);
-        return rc;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        for (int rep = 0; rep < 2; ++rep) {
+            CKH(hipEventRecord(c.ev_chain[0][0], c.stream));
+            hipLaunchKernelGGL(k_place_walk, dim3((lanes + 255) / 256), dim3(256), 0, c.stream,
+                               static_cast<uint64_t*>(cand[i]), pitch, static_cast<uint32_t>(pitch), lanes, steps, sink);
+            CKH(hipGetLastError());
+            CKH(hipEventRecord(c.ev_chain[0][1], c.stream));
+            CKH(hipEventSynchronize(c.ev_chain[0][1]));
+            CKH(hipEventElapsedTime(&ms[i], c.ev_chain[0][0], c.ev_chain[0][1]));
+        }
     }
     std::vector<int> order(cand.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ms[x] < ms[y]; });
     for (size_t i = 2; i < order.size(); ++i) CKH(hipFree(cand[order[i]]));
-    c.ws = Workspace{cand[order[0]], need};
-    c.ws2 = Workspace{cand[order[1]], need};
-    c.placed_for = need;
+    c.tabs[0] = Workspace{cand[order[0]], tabs};
+    c.tabs[1] = Workspace{cand[order[1]], tabs};
+    c.placed_for = tabs;
     c.place_ms = ms;
     c.place_kept[0] = order[0];
     c.place_kept[1] = order[1];
-    forget_tables(c);  // the search passes wrote the tables
+    forget_tables(c);
     return CK_OK;
 }
 
@@ -1604,7 +1616,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
                         lane);
         if (rc) return rc;
         placed = placeable(c, d);
-        rc = placed ? place_buffers(c, p, rk, first, d) : ensure_ws(c, d.nbuf * d.h.buffer);
+        rc = placed ? place_buffers(c, d) : ensure_ws(c, d.nbuf * d.h.buffer);
         if (rc != CK_ENOMEM || attempt == 8 || d.chunk == 1) break;
         c.budget /= 2;
     }
@@ -1617,13 +1629,13 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
     const bool bgh = p.mode == CK_MODE_BGH25;
     PassArgs args[2];
     for (uint32_t b = 0; b < nbuf; ++b) {
-        char* base = placed && b == 1 ? static_cast<char*>(c.ws2.base) : static_cast<char*>(c.ws.base) + b * h.buffer;
+        char* base = static_cast<char*>(c.ws.base) + b * (placed ? h.off_tables : h.buffer);
         PassArgs& a = args[b];
         a.S = make_shape(p);
         std::memcpy(a.bk.w, rk, sizeof a.bk.w);
         a.map = base + h.off_map;
         a.map_stride = h.map_stride;
-        a.tables = base + h.off_tables;
+        a.tables = placed ? static_cast<char*>(c.tabs[b].base) : base + h.off_tables;
         a.table_bytes = h.table_bytes;
         a.walks = reinterpret_cast<uint64_t*>(base + h.off_walk);
         a.rnds = reinterpret_cast<uint64_t*>(base + h.off_rnd);
@@ -1994,7 +2006,7 @@ void ck_ctx_destroy(ck_ctx* ctx) {
         g.drop();
         if (g.aux) (void)hipFree(g.aux);
     }
-    for (Workspace* w : {&ctx->c.ws, &ctx->c.ws2}) {
+    for (Workspace* w : {&ctx->c.ws, &ctx->c.tabs[0], &ctx->c.tabs[1]}) {
         if (!w->base) continue;
         if (ctx->c.ws_graphed)
             retire_ws(ctx->c.device, *w);
@@ -2037,11 +2049,11 @@ int ck_ctx_set_place_search(ck_ctx* ctx, uint32_t candidates) {
     return CK_OK;
 }
 
-int ck_ctx_place_report(ck_ctx* ctx, float* chain_ms, uint32_t cap, uint32_t* n, int32_t kept[2]) {
-    if (!ctx || !n || !kept || (cap && !chain_ms)) return fail(CK_EINVAL, "null argument");
+int ck_ctx_place_report(ck_ctx* ctx, float* walk_ms, uint32_t cap, uint32_t* n, int32_t kept[2]) {
+    if (!ctx || !n || !kept || (cap && !walk_ms)) return fail(CK_EINVAL, "null argument");
     const Ctx& c = ctx->c;
     *n = static_cast<uint32_t>(c.place_ms.size());
-    for (uint32_t i = 0; i < std::min(cap, *n); ++i) chain_ms[i] = c.place_ms[i];
+    for (uint32_t i = 0; i < std::min(cap, *n); ++i) walk_ms[i] = c.place_ms[i];
     kept[0] = c.placed_for ? c.place_kept[0] : -1;
     kept[1] = c.placed_for ? c.place_kept[1] : -1;
     return CK_OK;

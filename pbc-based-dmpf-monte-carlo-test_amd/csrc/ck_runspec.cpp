@@ -130,7 +130,7 @@ const Rule kRules[] = {
     {[](const RunSpec& s) { return !one_of(s.tier, {"auto", "hbm", "lds", "lane"}); },
      "--tier is auto, hbm, lds or lane"},
     {[](const RunSpec& s) { return s.place_search == 1 || s.place_search > 64; },
-     "--place-search is 0 (none) or 2..64 candidate buffers"},
+     "--place-search is 0 (none) or 2..64 candidate table regions"},
     {[](const RunSpec& s) { return s.table_layout == "shared" && s.global_table; },
      "--table-layout shared and --global-table exclude each other"},
     {[](const RunSpec& s) { return s.table_layout == "shared" && s.prp_cache == "off"; },
@@ -146,7 +146,7 @@ std::string usage_text() {
            "  placement:        [--scratch-mib MiB] [--prp-cache auto|on|off] [--table-layout auto|global|shared]\n"
            "                    [--shared-variant auto|round|map] [--global-table] [--tier auto|hbm|lds|lane]\n"
            "                    [--block-threads 64|128|256] [--warps-per-block N] [--tune] [--graph]\n"
-           "                    [--place-search 16]\n"
+           "                    [--place-search 32]\n"
            "  extension shape:  [--k 2..4] [--buckets M] [--kick N] [--stash S]\n";
 }
 

@@ -257,11 +257,11 @@ class Context:
         _check(LIB.ck_ctx_set_layout(self.h, layout))
 
     def set_place_search(self, candidates: int):
-        """Pass-buffer placement search: candidates per search (0 = off)."""
+        """Table-region placement search: candidates per search (0 = off)."""
         _check(LIB.ck_ctx_set_place_search(self.h, candidates))
 
     def place_report(self):
-        """The last placement search: (chain ms per candidate, kept indices)."""
+        """The last placement search: (walk ms per candidate, kept indices)."""
         ms = (C.c_float * 64)()
         n = C.c_uint32()
         kept = (C.c_int32 * 2)()

@@ -531,7 +531,7 @@ void test_parse() {
     EXPECT(d.input_mode == "sequential" && d.workers == 0 && d.scratch_mib == 256 && !d.scratch_given);
     EXPECT(d.block_threads == 256 && d.warps_per_block == 0 && d.prp_cache == "auto");
     EXPECT(d.table_layout == "auto" && d.shared_variant == "auto" && d.k == 3 && d.stash == 0);
-    EXPECT(d.place_search == 16 && parse({"--place-search", "0"}).place_search == 0);
+    EXPECT(d.place_search == 32 && parse({"--place-search", "0"}).place_search == 0);
     const RunSpec x = parse({"--pnum", "0x40", "--seed", "18446744073709551615", "--scratch-mib", "2048",
                              "--status", "--global-table", "--workers", "3"});
     EXPECT(x.pnum == 64 && x.seed == ~0ull && x.scratch_mib == 2048 && x.scratch_given && x.status_only);
