@@ -17,14 +17,16 @@
 // For each lane count it prints one line per form:
 //   read_write lanes=L table_bytes=B steps=S ms=T steps_per_s=R
 //   read_only  ...
+//   read_only_w4 ...   (four independent walks per lane, S steps in all)
 // The rate belongs to the memory the tables land in (the chain's passes run
 // 15 % faster in some allocations than in others), so with --search K it then
 // allocates K buffers for 65,536 lanes, all held at once, runs both forms in
 // each and prints
 //   search_read_write buffer=i lanes=65536 ... steps_per_s=R
 //   search_read_only  ...
-// bench.py takes the best read_only rate over all lines as the ceiling that it
-// sets the chain's DRAM request rate against.
+//   search_read_only_w4 ...
+// bench.py takes the best read-only rate over all lines (either form) as the
+// ceiling that it sets the chain's DRAM request rate against.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -50,21 +52,33 @@ __global__ __launch_bounds__(256) void k_fill(uint64_t* __restrict__ buf, uint64
         buf[i] = mix64(i + salt);
 }
 
-template <bool WRITE>
+// W independent walks per lane over the lane's table, steps / W steps each
+// (W = 1: the chain's single dependent path; W = 4: four loads in flight per
+// lane, the read-only rate with latency taken out of the way).
+template <bool WRITE, int W>
 __global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64_t pitch, uint32_t slots,
                                               uint32_t lanes, uint32_t steps, unsigned long long* __restrict__ sink) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= lanes) return;
     uint64_t* t = buf + uint64_t(lane) * pitch;
-    uint32_t ctr = lane * kGolden32;
-    uint32_t i = __umulhi(static_cast<uint32_t>(mix64(lane)), slots);
+    uint32_t ctr[W], i[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        ctr[w] = (lane * W + w) * kGolden32;
+        i[w] = __umulhi(static_cast<uint32_t>(mix64(uint64_t(lane) * W + w)), slots);
+    }
     uint64_t acc = 0;
-    for (uint32_t s = 0; s < steps; ++s) {
-        const uint64_t v = t[i];
-        if constexpr (WRITE) t[i] = v ^ ctr;
-        acc += v;
-        i = __umulhi(static_cast<uint32_t>(v) ^ ctr, slots);
-        ctr += kGolden32;
+    for (uint32_t s = 0; s < steps / W; ++s) {
+        uint64_t v[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) v[w] = t[i[w]];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if constexpr (WRITE) t[i[w]] = v[w] ^ ctr[w];
+            acc += v[w];
+            i[w] = __umulhi(static_cast<uint32_t>(v[w]) ^ ctr[w], slots);
+            ctr[w] += kGolden32;
+        }
     }
     if (acc == 0x5eedull) sink[0] = acc;  // keeps the loads live
 }
@@ -80,31 +94,34 @@ __global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64
 
 }  // namespace
 
-// Both forms at `lanes` lanes over buf; prints one line per form with `tag`.
+// The three forms at `lanes` lanes over buf; prints one line per form with `tag`.
 static int measure(const char* tag, int buffer, uint64_t* buf, uint64_t pitch, uint32_t slots, uint32_t lanes,
                    uint64_t table, uint32_t steps, unsigned long long* sink, hipEvent_t a, hipEvent_t b) {
     const dim3 grid((lanes + 255) / 256), block(256);
-    for (int form = 0; form < 2; ++form) {
+    static const char* const kForm[3] = {"read_write", "read_only", "read_only_w4"};
+    for (int form = 0; form < 3; ++form) {
         float ms = 0;
         for (int rep = 0; rep < 2; ++rep) {  // the first run warms the TLB and clocks
             CHECK(hipEventRecord(a));
             if (form == 0)
-                hipLaunchKernelGGL(k_walk<true>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+                hipLaunchKernelGGL((k_walk<true, 1>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+            else if (form == 1)
+                hipLaunchKernelGGL((k_walk<false, 1>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
             else
-                hipLaunchKernelGGL(k_walk<false>, grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+                hipLaunchKernelGGL((k_walk<false, 4>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
             CHECK(hipGetLastError());
             CHECK(hipEventRecord(b));
             CHECK(hipEventSynchronize(b));
             CHECK(hipEventElapsedTime(&ms, a, b));
         }
-        const double rate = double(lanes) * steps / (ms / 1e3);
+        const uint32_t done = form == 2 ? steps / 4 * 4 : steps;
+        const double rate = double(lanes) * done / (ms / 1e3);
         if (buffer >= 0)
             std::printf("%s%s buffer=%d lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n", tag,
-                        form == 0 ? "read_write" : "read_only", buffer, lanes, (unsigned long long)table, steps, ms,
-                        rate);
+                        kForm[form], buffer, lanes, (unsigned long long)table, done, ms, rate);
         else
-            std::printf("%s%s lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n", tag,
-                        form == 0 ? "read_write" : "read_only", lanes, (unsigned long long)table, steps, ms, rate);
+            std::printf("%s%s lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n", tag, kForm[form],
+                        lanes, (unsigned long long)table, done, ms, rate);
         std::fflush(stdout);
     }
     return 0;
