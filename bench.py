@@ -802,10 +802,19 @@ def main():
     }
     rec["roofline"]["random_access_ceiling"] = random_access_ceiling(probes_per_launch / chain_avg_s,
                                                                      prof["requests"] if prof else None)
-    # The pass-buffer placement search of this context (chain ms of one pass in
-    # each candidate buffer, the two kept).
+    # The table placement of this context: by region (pieces per region, the
+    # regions of each pass buffer's three pieces, walk ms of each candidate
+    # table range, the two kept) or by search (walk ms per candidate, the two
+    # kept).
     ms, kept = ctx.place_report()
-    rec["config"]["placement_search"] = {"candidate_walk_ms": [round(x, 2) for x in ms], "kept": list(kept)}
+    regions, b0, b1 = ctx.place_regions()
+    if regions:
+        rec["config"]["placement_regions"] = {
+            "pieces": len(regions), "per_region": [regions.count(r) for r in range(3)],
+            "buffer0_regions": [regions[i] for i in b0], "buffer1_regions": [regions[i] for i in b1],
+            "candidate_walk_ms": [round(x, 2) for x in ms], "kept": list(kept)}
+    else:
+        rec["config"]["placement_search"] = {"candidate_walk_ms": [round(x, 2) for x in ms], "kept": list(kept)}
     ok = True
     if not args.no_cpu:
         rec["cpu_baseline"], ok = cpu_baseline(ctx, ck, p, args.cpu_seconds)

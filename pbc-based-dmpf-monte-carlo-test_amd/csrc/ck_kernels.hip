@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <mutex>
 #include <cstring>
@@ -355,6 +356,28 @@ __global__ __launch_bounds__(256) void k_place_walk(uint64_t* __restrict__ base,
     if (acc == 0x5eedull) sink[0] = acc;
 }
 
+// Region placement: the same walk with 2 x `half` lanes, the first half over
+// tables of `slots` words at a, the second half at b (half x slots words at
+// each), so that its rate tells whether a and b lie in one memory region.
+__global__ __launch_bounds__(256) void k_place_pair(uint64_t* __restrict__ a, uint64_t* __restrict__ b, uint32_t half,
+                                                    uint32_t slots, uint32_t steps,
+                                                    unsigned long long* __restrict__ sink) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= 2 * half) return;
+    uint64_t* t = (lane < half ? a : b) + uint64_t(lane < half ? lane : lane - half) * slots;
+    uint32_t ctr = lane * 0x9E3779B9u;
+    uint32_t i = __umulhi(lane * 0x85EBCA6Bu + 0x27D4EB2Fu, slots);
+    uint64_t acc = 0;
+    for (uint32_t s = 0; s < steps; ++s) {
+        const uint64_t v = t[i];
+        t[i] = v ^ ctr;
+        acc += v;
+        i = __umulhi(static_cast<uint32_t>(v) ^ ctr, slots);
+        ctr += 0x9E3779B9u;
+    }
+    if (acc == 0x5eedull) sink[0] = acc;
+}
+
 // LDS tier: a persistent wave per trial at a time.  Wave region layout:
 // [round cache 8<<half u16][map pnum*K M, unless ROUND][table buckets E].
 // The chain is the trial API's insert_core_k (include/cuckoo_trial.hpp) on
@@ -476,6 +499,16 @@ __global__ __launch_bounds__(64) void k_aes_kat(uint32_t* out, uint32_t tab_shif
 struct Workspace {
     void* base = nullptr;
     size_t bytes = 0;
+};
+
+// The tables of one pass buffer, held apart from it by a placed plan: one
+// hipMalloc (pieces empty), or `pieces` physical allocations of `piece` bytes
+// each mapped one after another at `base` (virtual-memory API).
+struct TableRange {
+    void* base = nullptr;
+    size_t bytes = 0;
+    size_t piece = 0;
+    std::vector<hipMemGenericAllocationHandle_t> pieces;
 };
 
 // The workspace of ck_launch_static: a file-scope device array, so that
@@ -611,14 +644,17 @@ struct Ctx {
     hipEvent_t ev_build[2][2] = {};  // [buffer][start, end]
     hipEvent_t ev_chain[2][2] = {};
     Workspace ws;
-    // Placement search (see place_buffers): the table regions of the two pass
-    // buffers are their own allocations, the two fastest of place_cands
-    // candidates; ws then holds the rest of both pass buffers.
-    Workspace tabs[2];
+    // Table placement (see place_buffers): the tables of the two pass buffers
+    // are held apart from them, spread over the device's memory regions (or,
+    // where that cannot be done, the two fastest of place_cands candidate
+    // allocations); ws then holds the rest of both pass buffers.
+    TableRange tabs[2];
     size_t placed_for = 0;           // table-region size tabs were placed for (0: not placed)
-    uint32_t place_cands = 0;        // candidates per search, as many as the budget holds (0: no search)
-    std::vector<float> place_ms;     // last search: walk ms of each candidate, in allocation order
-    int place_kept[2] = {-1, -1};    // candidates kept as pass buffers 0 and 1
+    uint32_t place_cands = 0;        // on/off, and candidates per fallback search (0: no placement)
+    std::vector<float> place_ms;     // last placement: walk ms per candidate or piece, in allocation order
+    int place_kept[2] = {-1, -1};    // candidates (or first pieces) kept for pass buffers 0 and 1
+    std::vector<int32_t> place_region;   // region placement: each piece's region (empty: search)
+    int32_t place_used[6] = {-1, -1, -1, -1, -1, -1};  // pieces mapped as buffer 0's, then buffer 1's tables
     uint32_t tier = CK_TIER_AUTO;
     uint32_t layout = CK_LAYOUT_AUTO;
     uint32_t chunk_cap = 0;
@@ -957,18 +993,36 @@ static void forget_tables(Ctx& c) {
 // Frees (or, after graph use, retires) the workspace and a placed second pass
 // buffer.  Graphs whose kernel nodes point into them are destroyed first (they
 // could never be replayed against a new workspace).
+static int free_tables(Ctx& c, TableRange& t) {
+    if (!t.base) return CK_OK;
+    if (t.pieces.empty()) {
+        if (c.ws_graphed)
+            retire_ws(c.device, Workspace{t.base, t.bytes});
+        else
+            CKH(hipFree(t.base));
+    } else {  // never in a graph (placed plans are not captured); unmapped once idle
+        CKH(hipDeviceSynchronize());
+        for (size_t i = 0; i < t.pieces.size(); ++i) CKH(hipMemUnmap(static_cast<char*>(t.base) + i * t.piece, t.piece));
+        for (hipMemGenericAllocationHandle_t h : t.pieces) CKH(hipMemRelease(h));
+        CKH(hipMemAddressFree(t.base, t.bytes));
+    }
+    t = TableRange{};
+    return CK_OK;
+}
+
 static int release_ws(Ctx& c) {
     forget_tables(c);
     for (HbmGraph& g : c.graphs)
         if (g.exec && g.ws == c.ws.base) g.drop();
-    for (Workspace* w : {&c.ws, &c.tabs[0], &c.tabs[1]}) {
-        if (!w->base) continue;
+    if (c.ws.base) {
         if (c.ws_graphed)
-            retire_ws(c.device, *w);
+            retire_ws(c.device, c.ws);
         else
-            CKH(hipFree(w->base));
-        *w = Workspace{};
+            CKH(hipFree(c.ws.base));
+        c.ws = Workspace{};
     }
+    for (TableRange& t : c.tabs)
+        if (int rc = free_tables(c, t)) return rc;
     c.ws_graphed = false;
     c.placed_for = 0;
     return CK_OK;
@@ -1503,33 +1557,324 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
     return CK_OK;
 }
 
-// Placement search.  The chain rate of a pass belongs to the memory its
-// tables land in: at n = 2^14 (65,536 chains) the same pass chained in 93-95
-// ms in most pass buffers and in 73-79 ms in some, the same figure every time
-// in a given buffer, and a dependent random read-write walk shows the same two
-// rates (20.5 and 24.6 G steps/s) per allocation, from 6 GiB to 64 GiB, with
-// fast allocations rarer the larger they are.  Which memory is fast cannot be
-// asked for (neither the contiguous-allocation flag nor the virtual-memory API
-// changes the odds), so a double-buffered plan with large pass buffers gives
-// each pass buffer's tables an allocation of their own: it allocates up to
-// place_cands candidate table regions, times k_place_walk over each (the
-// chains' access pattern, a few ms) and keeps the two fastest.  The rest of
-// both pass buffers (candidate maps, per-slot words) shares one allocation;
-// with fast tables the chains ran at the fast rate wherever it landed.  The
-// search reruns only when a plan needs tables of another size, so it is for
-// contexts that run many passes of one shape (ck_ctx_set_place_search turns
-// it on; the CLI and the headline bench do).  Results do not depend on it:
-// every pass computes the same trials in any buffer.
-// Searched plans: passes of at least kPlaceMinChains chains (a walk over
-// fewer lanes does not load the random-access path) in buffers of at least 1 GiB.
+// Table placement.  The chain rate of a pass belongs to the memory its tables
+// land in.  The device's memory falls into three regions of about 96 GB
+// (found by pairing 1.5-2 GiB allocations: a dependent random read-write walk
+// with half its lanes in each of two allocations runs at 22.7 G steps/s when
+// both lie in one region and at 29.1 G when they lie in two, and every
+// allocation pairs slow with one third of the others), and random accesses
+// confined to one region run slower than accesses spread over several: 65,536
+// lanes walking 96 KB tables made 20.5 G steps/s in one region, 24.7 G split
+// 2:1 over two, 27.3 G split evenly over three.  Allocations fill the regions
+// in turn, so an ordinary buffer of a few GB lies in one region (the chains'
+// 93-95 ms per pass at n = 2^14), and one that happens to straddle a boundary
+// runs fast (73-79 ms): neither the contiguous-allocation flag nor any
+// allocation size changes that.
+//
+// A double-buffered plan with large pass buffers (placeable) therefore holds
+// the tables of each pass buffer apart from it.  place_regions allocates the
+// tables in pieces of a third (virtual-memory API), as many as the budget
+// holds, sorts them into regions by pair walks (k_place_pair) and maps one
+// piece from each of three regions (2:1 over two where only two show) into
+// each pass buffer's table range.  Where pieces would be under kRegionMinPiece
+// (the pair walk needs 768 MiB per piece to stay out of the caches) or fewer
+// than two regions show, place_buffers falls back to a search: up to
+// place_cands candidate table allocations, k_place_walk (the chains' access
+// pattern) over each, the two fastest kept.  The rest of both pass buffers
+// (candidate maps, per-slot words) shares one allocation; its placement did
+// not change the chain.  Placement reruns only when a plan needs tables of
+// another size, so it is for contexts that run many passes of one shape
+// (ck_ctx_set_place_search turns it on; the CLI and the headline bench do).
+// Results do not depend on it: every pass computes the same trials anywhere.
+// Placed plans: passes of at least kPlaceMinChains chains (a walk over fewer
+// lanes does not load the random-access path) in buffers of at least 1 GiB.
 static constexpr size_t kPlaceMinBuffer = size_t(1) << 30;
 static constexpr uint32_t kPlaceWalkSteps = 4096;
 static constexpr uint32_t kPlaceMinChains = 1u << 14;
+static constexpr size_t kRegionMinPiece = size_t(1) << 30;
+static constexpr size_t kRegionPieceAlign = size_t(2) << 20;
+static constexpr uint32_t kRegionMaxPieces = 192;
+static constexpr uint32_t kRegionPairLanes = 65536, kRegionPairSlots = 3072, kRegionPairSteps = 1024;
+static constexpr float kRegionModes = 1.12f;  // slow / fast pair time at least this: two regions seen
+static constexpr uint32_t kRegionCands = 6;   // candidate table ranges timed
+static constexpr float kRegionNear = 1.06f;   // within this factor of a mode: in that mode
+static constexpr uint32_t kRegionBatch = 8;   // pieces allocated between classifications
+
 static bool placeable(const Ctx& c, const HbmDecision& d) {
     return c.place_cands >= 2 && !c.ws_static && !c.graph_on && d.nbuf == 2 && d.h.nwin == 1 &&
            d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer &&
            c.budget >= 2 * d.h.off_tables + 3 * (d.h.buffer - d.h.off_tables);
 }
+
+// Event-timed launch of `launch` on the chain stream: once to warm the
+// translations, once timed.
+template <typename F>
+static int timed_twice(Ctx& c, F&& launch, float* ms) {
+    for (int rep = 0; rep < 2; ++rep) {
+        CKH(hipEventRecord(c.ev_chain[0][0], c.stream));
+        launch();
+        CKH(hipGetLastError());
+        CKH(hipEventRecord(c.ev_chain[0][1], c.stream));
+        CKH(hipEventSynchronize(c.ev_chain[0][1]));
+        CKH(hipEventElapsedTime(ms, c.ev_chain[0][0], c.ev_chain[0][1]));
+    }
+    return CK_OK;
+}
+
+// Two modes among the times t[idx]: 2-means on their logarithms (so a few
+// pieces in between, such as the reference's neighbours or pieces straddling
+// a boundary, do not set the modes), each mode the median of its cluster.
+// False unless both clusters hold two or more and the slow mode is at least
+// kRegionModes times the fast one.
+static bool split_modes(const std::vector<float>& t, const std::vector<size_t>& idx, float* fast, float* slow) {
+    std::vector<float> v;
+    for (size_t j : idx)
+        if (t[j] > 0) v.push_back(std::log(t[j]));
+    if (v.size() < 4) return false;
+    std::sort(v.begin(), v.end());
+    float lo = v.front(), hi = v.back();
+    size_t cut = 0;  // v[0, cut) fast, v[cut, n) slow
+    for (int it = 0; it < 32; ++it) {
+        const float mid = 0.5f * (lo + hi);
+        cut = static_cast<size_t>(std::upper_bound(v.begin(), v.end(), mid) - v.begin());
+        if (cut == 0 || cut == v.size()) return false;
+        float sl = 0, sh = 0;
+        for (size_t i = 0; i < v.size(); ++i) (i < cut ? sl : sh) += v[i];
+        const float nlo = sl / cut, nhi = sh / (v.size() - cut);
+        if (nlo == lo && nhi == hi) break;
+        lo = nlo;
+        hi = nhi;
+    }
+    if (cut < 2 || v.size() - cut < 2) return false;
+    *fast = std::exp(v[cut / 2]);
+    *slow = std::exp(v[cut + (v.size() - cut) / 2]);
+    return *slow >= *fast * kRegionModes;
+}
+
+// Region placement (see above).  *placed stays false, with nothing allocated
+// beyond ws, when it does not apply: pieces too small, no virtual-memory
+// support, fewer than two regions seen, or too few pieces in them.  Pieces
+// are allocated in batches and sorted as they come, and allocation stops once
+// every region holds kRegionCands of them: a device whose memory was freed in
+// scattered pieces shows all three regions within a few batches (and makes
+// each new piece wait for the driver to clear it), a fresh one only after the
+// first two regions are used up.
+static int place_regions(Ctx& c, const HbmDecision& d, bool* placed) {
+    *placed = false;
+    const size_t rest = d.h.off_tables, tabs = d.h.buffer - d.h.off_tables;
+    const size_t P = ((tabs + 2) / 3 + kRegionPieceAlign - 1) & ~(kRegionPieceAlign - 1);
+    const size_t room = c.budget > 2 * rest ? c.budget - 2 * rest : 0;
+    const size_t maxp = std::min<size_t>(kRegionMaxPieces, room / P);
+    if (P < kRegionMinPiece || maxp < 6) return CK_OK;
+    hipMemAllocationProp prop{};
+    prop.type = hipMemAllocationTypePinned;
+    prop.location.type = hipMemLocationTypeDevice;
+    prop.location.id = c.device;
+    hipMemAccessDesc acc{};
+    acc.location = prop.location;
+    acc.flags = hipMemAccessFlagsProtReadWrite;
+    char* pool = nullptr;
+    if (hipMemAddressReserve(reinterpret_cast<void**>(&pool), maxp * P, 0, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return CK_OK;
+    }
+    std::vector<hipMemGenericAllocationHandle_t> h;
+    size_t mapped = 0, ready = 0;  // pieces mapped into the pool; of them, accessible
+    TableRange out[2];
+    int rc = CK_OK;
+    // Unmaps the pool and releases every piece not in out[].
+    auto drop_pool = [&]() {
+        (void)hipDeviceSynchronize();
+        for (size_t j = 0; j < mapped; ++j) (void)hipMemUnmap(pool + j * P, P);
+        (void)hipMemAddressFree(pool, maxp * P);
+        for (hipMemGenericAllocationHandle_t x : h) {
+            bool kept = false;
+            for (const TableRange& t : out)
+                for (hipMemGenericAllocationHandle_t y : t.pieces) kept |= x == y;
+            if (!kept) (void)hipMemRelease(x);
+        }
+    };
+    // Pair walks: half the lanes in piece a, half in piece b, 24 KB tables
+    // (768 MiB of each piece).
+    auto* sink = reinterpret_cast<unsigned long long*>(c.ws.base);  // the first pass's counters, cleared per pass
+    auto pair_ms = [&](size_t a, size_t b, float* ms) {
+        return timed_twice(
+            c,
+            [&]() {
+                hipLaunchKernelGGL(k_place_pair, dim3(kRegionPairLanes / 256), dim3(256), 0, c.stream,
+                                   reinterpret_cast<uint64_t*>(pool + a * P), reinterpret_cast<uint64_t*>(pool + b * P),
+                                   kRegionPairLanes / 2, kRegionPairSlots, kRegionPairSteps, sink);
+            },
+            ms);
+    };
+    // t0: each piece paired with piece 0; t1: each piece of another region
+    // than piece 0's paired with ref1, the one most clearly apart from piece 0.
+    std::vector<float> t0, t1;
+    std::vector<int32_t> region;
+    size_t ref1 = 0;
+    bool two = false, full = false;
+    std::vector<size_t> of[3];
+    while (rc == CK_OK && !full) {
+        const size_t from = ready;
+        for (size_t k = 0; k < kRegionBatch && ready < maxp; ++k) {
+            hipMemGenericAllocationHandle_t x{};
+            if (hipMemCreate(&x, P, &prop, 0) != hipSuccess) {  // the device is full (or has no such API)
+                (void)hipGetLastError();
+                full = true;
+                break;
+            }
+            h.push_back(x);
+            hipError_t e = hipMemMap(pool + ready * P, P, 0, x, 0);
+            if (e == hipSuccess) {
+                ++mapped;
+                e = hipMemSetAccess(pool + ready * P, P, &acc, 1);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                full = true;
+                break;
+            }
+            ++ready;
+        }
+        if (ready == maxp) full = true;
+        if (ready == from) break;
+        t0.resize(ready, 0.f);
+        t1.resize(ready, 0.f);
+        region.resize(ready, 0);
+        for (size_t j = std::max<size_t>(from, 1); j < ready && rc == CK_OK; ++j) rc = pair_ms(0, j, &t0[j]);
+        if (rc) break;
+        // Piece 0's region: near the slow mode; another: near the fast mode;
+        // in between: left out (-1).
+        std::vector<size_t> all;
+        for (size_t j = 1; j < ready; ++j) all.push_back(j);
+        float f0 = 0, s0 = 0;
+        two = split_modes(t0, all, &f0, &s0);
+        std::vector<size_t> others;
+        for (size_t j = 1; j < ready; ++j) {
+            region[j] = !two ? 0 : t0[j] >= s0 / kRegionNear ? 0 : t0[j] <= f0 * kRegionNear ? 1 : -1;
+            if (region[j] == 1) others.push_back(j);
+        }
+        if (two && !ref1 && !others.empty()) {
+            ref1 = others[0];
+            for (size_t j : others)
+                if (t0[j] < t0[ref1]) ref1 = j;
+        }
+        // A second split among the others, where it shows, is a third region.
+        std::vector<size_t> others1;
+        for (size_t j : others)
+            if (j != ref1) {
+                if (t1[j] == 0 && rc == CK_OK) rc = pair_ms(ref1, j, &t1[j]);
+                others1.push_back(j);
+            }
+        if (rc) break;
+        float f1 = 0, s1 = 0;
+        if (ref1 && split_modes(t1, others1, &f1, &s1))
+            for (size_t j : others1) region[j] = t1[j] >= s1 / kRegionNear ? 1 : t1[j] <= f1 * kRegionNear ? 2 : -1;
+        for (auto& v : of) v.clear();
+        for (size_t j = 0; j < ready; ++j)
+            if (region[j] >= 0) of[region[j]].push_back(j);
+        if (of[0].size() >= kRegionCands && of[1].size() >= kRegionCands && of[2].size() >= kRegionCands) break;
+    }
+    // Candidate table ranges of three pieces, pairwise disjoint: one piece per
+    // region, or (two classes) one of the first region and two of the rest,
+    // from either end of their allocation order.
+    std::vector<std::array<size_t, 3>> cand;
+    if (two && of[1].size() >= 2 && of[2].size() >= 2) {
+        const size_t k = std::min({size_t(kRegionCands), of[0].size(), of[1].size(), of[2].size()});
+        for (size_t i = 0; i < k; ++i) cand.push_back({of[0][i], of[1][i], of[2][i]});
+    } else if (two) {
+        std::vector<size_t> rest1 = of[1];
+        rest1.insert(rest1.end(), of[2].begin(), of[2].end());
+        std::sort(rest1.begin(), rest1.end());
+        const size_t k = std::min({size_t(kRegionCands), of[0].size(), rest1.size() / 2});
+        for (size_t i = 0; i < k; ++i) cand.push_back({of[0][i], rest1[i], rest1[rest1.size() - 1 - i]});
+    }
+    if (rc != CK_OK || cand.size() < 2) {
+        drop_pool();
+        return rc;
+    }
+    // Each candidate mapped into a test range and timed with the chains'
+    // access pattern at the plan's own geometry; the two fastest are kept.
+    std::vector<float> cms(cand.size(), 0.f);
+    void* test = nullptr;
+    if (hipMemAddressReserve(&test, 3 * P, 0, nullptr, 0) != hipSuccess) {
+        drop_pool();
+        return fail(CK_EDEVICE, "hipMemAddressReserve for a test table range");
+    }
+    const uint64_t pitch = d.h.table_bytes / 8;
+    for (size_t i = 0; i < cand.size() && rc == CK_OK; ++i) {
+        int maps = 0;
+        for (int r = 0; r < 3; ++r) {
+            if (hipMemMap(static_cast<char*>(test) + r * P, P, 0, h[cand[i][r]], 0) != hipSuccess) break;
+            ++maps;
+        }
+        if (maps < 3 || hipMemSetAccess(test, 3 * P, &acc, 1) != hipSuccess)
+            rc = fail(CK_EDEVICE, "mapping a test table range");
+        if (rc == CK_OK)
+            rc = timed_twice(
+                c,
+                [&]() {
+                    hipLaunchKernelGGL(k_place_walk, dim3((d.chunk + 255) / 256), dim3(256), 0, c.stream,
+                                       static_cast<uint64_t*>(test), pitch, static_cast<uint32_t>(pitch), d.chunk,
+                                       kPlaceWalkSteps, sink);
+                },
+                &cms[i]);
+        (void)hipDeviceSynchronize();
+        for (int r = 0; r < maps; ++r) (void)hipMemUnmap(static_cast<char*>(test) + r * P, P);
+    }
+    (void)hipMemAddressFree(test, 3 * P);
+    std::vector<int> order(cand.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cms[x] < cms[y]; });
+    int32_t used[6];
+    for (int b = 0; b < 2; ++b)
+        for (int r = 0; r < 3; ++r) used[3 * b + r] = static_cast<int32_t>(cand[order[b]][r]);
+    if (rc != CK_OK) {
+        drop_pool();
+        return rc;
+    }
+    for (int b = 0; b < 2 && rc == CK_OK; ++b) {
+        TableRange& t = out[b];
+        void* va = nullptr;
+        if (hipMemAddressReserve(&va, 3 * P, 0, nullptr, 0) != hipSuccess) {
+            rc = fail(CK_EDEVICE, "hipMemAddressReserve for the table range");
+            break;
+        }
+        t.base = va;
+        t.bytes = 3 * P;
+        t.piece = P;
+        for (int i = 0; i < 3 && rc == CK_OK; ++i) {
+            if (hipMemMap(static_cast<char*>(va) + i * P, P, 0, h[used[3 * b + i]], 0) != hipSuccess)
+                rc = fail(CK_EDEVICE, "hipMemMap of a table piece");
+            else
+                t.pieces.push_back(h[used[3 * b + i]]);
+        }
+        if (rc == CK_OK && hipMemSetAccess(va, 3 * P, &acc, 1) != hipSuccess) rc = fail(CK_EDEVICE, "hipMemSetAccess");
+    }
+    if (rc != CK_OK) {
+        (void)hipDeviceSynchronize();
+        for (TableRange& t : out) {
+            for (size_t i = 0; i < t.pieces.size(); ++i) (void)hipMemUnmap(static_cast<char*>(t.base) + i * P, P);
+            if (t.base) (void)hipMemAddressFree(t.base, t.bytes);
+            t = TableRange{};
+        }
+        drop_pool();
+        return rc;
+    }
+    drop_pool();
+    c.tabs[0] = std::move(out[0]);
+    c.tabs[1] = std::move(out[1]);
+    c.placed_for = tabs;
+    c.place_ms = cms;
+    c.place_region = region;
+    std::copy(used, used + 6, c.place_used);
+    c.place_kept[0] = order[0];
+    c.place_kept[1] = order[1];
+    forget_tables(c);
+    *placed = true;
+    return CK_OK;
+}
+
 static int place_buffers(Ctx& c, const HbmDecision& d) {
     const size_t rest = d.h.off_tables, tabs = d.h.buffer - d.h.off_tables;
     if (c.placed_for == tabs && c.ws.bytes >= 2 * rest && c.tabs[0].bytes >= tabs && c.tabs[1].bytes >= tabs)
@@ -1543,6 +1888,9 @@ static int place_buffers(Ctx& c, const HbmDecision& d) {
     }
     if (e != hipSuccess) return fail(CK_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
     c.ws.bytes = 2 * rest;
+    bool placed = false;
+    if (int rc = place_regions(c, d, &placed)) return rc;
+    if (placed) return CK_OK;
     const size_t room = c.budget > 2 * rest ? c.budget - 2 * rest : 0;
     const uint32_t want = static_cast<uint32_t>(std::min<size_t>(c.place_cands, room / tabs));
     std::vector<void*> cand;
@@ -1565,30 +1913,39 @@ static int place_buffers(Ctx& c, const HbmDecision& d) {
     }
     // Each candidate: the chains' access pattern over its table regions (one
     // lane per trial slot, a dependent random read and write of 8-byte words
-    // of the lane's own table), once to warm the translations, once timed.
+    // of the lane's own table).
     const uint64_t pitch = d.h.table_bytes / 8;
-    const uint32_t lanes = d.chunk, steps = kPlaceWalkSteps;
-    auto* sink = reinterpret_cast<unsigned long long*>(c.ws.base);  // the first pass's counters, cleared per pass
+    const uint32_t lanes = d.chunk;
+    auto* sink = reinterpret_cast<unsigned long long*>(c.ws.base);
     std::vector<float> ms(cand.size(), 0.f);
     for (size_t i = 0; i < cand.size(); ++i) {
-        for (int rep = 0; rep < 2; ++rep) {
-            CKH(hipEventRecord(c.ev_chain[0][0], c.stream));
-            hipLaunchKernelGGL(k_place_walk, dim3((lanes + 255) / 256), dim3(256), 0, c.stream,
-                               static_cast<uint64_t*>(cand[i]), pitch, static_cast<uint32_t>(pitch), lanes, steps, sink);
-            CKH(hipGetLastError());
-            CKH(hipEventRecord(c.ev_chain[0][1], c.stream));
-            CKH(hipEventSynchronize(c.ev_chain[0][1]));
-            CKH(hipEventElapsedTime(&ms[i], c.ev_chain[0][0], c.ev_chain[0][1]));
+        const int rc = timed_twice(
+            c,
+            [&]() {
+                hipLaunchKernelGGL(k_place_walk, dim3((lanes + 255) / 256), dim3(256), 0, c.stream,
+                                   static_cast<uint64_t*>(cand[i]), pitch, static_cast<uint32_t>(pitch), lanes,
+                                   kPlaceWalkSteps, sink);
+            },
+            &ms[i]);
+        if (rc) {
+            (void)hipDeviceSynchronize();
+            for (void* x : cand) (void)hipFree(x);
+            return rc;
         }
     }
     std::vector<int> order(cand.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ms[x] < ms[y]; });
     for (size_t i = 2; i < order.size(); ++i) CKH(hipFree(cand[order[i]]));
-    c.tabs[0] = Workspace{cand[order[0]], tabs};
-    c.tabs[1] = Workspace{cand[order[1]], tabs};
+    for (int b = 0; b < 2; ++b) {
+        c.tabs[b] = TableRange{};
+        c.tabs[b].base = cand[order[b]];
+        c.tabs[b].bytes = tabs;
+    }
     c.placed_for = tabs;
     c.place_ms = ms;
+    c.place_region.clear();
+    std::fill(c.place_used, c.place_used + 6, -1);
     c.place_kept[0] = order[0];
     c.place_kept[1] = order[1];
     forget_tables(c);
@@ -2006,13 +2363,13 @@ void ck_ctx_destroy(ck_ctx* ctx) {
         g.drop();
         if (g.aux) (void)hipFree(g.aux);
     }
-    for (Workspace* w : {&ctx->c.ws, &ctx->c.tabs[0], &ctx->c.tabs[1]}) {
-        if (!w->base) continue;
+    if (ctx->c.ws.base) {
         if (ctx->c.ws_graphed)
-            retire_ws(ctx->c.device, *w);
+            retire_ws(ctx->c.device, ctx->c.ws);
         else
-            (void)hipFree(w->base);
+            (void)hipFree(ctx->c.ws.base);
     }
+    for (TableRange& t : ctx->c.tabs) (void)free_tables(ctx->c, t);
     for (auto& ev : ctx->c.ev)
         if (ev) (void)hipEventDestroy(ev);
     for (int b = 0; b < 2; ++b)
@@ -2056,6 +2413,16 @@ int ck_ctx_place_report(ck_ctx* ctx, float* walk_ms, uint32_t cap, uint32_t* n, 
     for (uint32_t i = 0; i < std::min(cap, *n); ++i) walk_ms[i] = c.place_ms[i];
     kept[0] = c.placed_for ? c.place_kept[0] : -1;
     kept[1] = c.placed_for ? c.place_kept[1] : -1;
+    return CK_OK;
+}
+
+int ck_ctx_place_regions(ck_ctx* ctx, int32_t* region, uint32_t cap, uint32_t* n, int32_t used[6]) {
+    if (!ctx || !n || !used || (cap && !region)) return fail(CK_EINVAL, "null argument");
+    const Ctx& c = ctx->c;
+    const bool on = c.placed_for && !c.place_region.empty();
+    *n = on ? static_cast<uint32_t>(c.place_region.size()) : 0;
+    for (uint32_t i = 0; i < std::min(cap, *n); ++i) region[i] = c.place_region[i];
+    for (int i = 0; i < 6; ++i) used[i] = on ? c.place_used[i] : -1;
     return CK_OK;
 }
 

@@ -118,6 +118,7 @@ def _load():
         "ck_ctx_set_window": (C.c_int, [vp, U32]),
         "ck_ctx_set_place_search": (C.c_int, [vp, U32]),
         "ck_ctx_place_report": (C.c_int, [vp, P(C.c_float), U32, P(U32), P(C.c_int32)]),
+        "ck_ctx_place_regions": (C.c_int, [vp, P(C.c_int32), U32, P(U32), P(C.c_int32)]),
         "ck_ctx_set_layout": (C.c_int, [vp, U32]),
         "ck_ctx_set_graph": (C.c_int, [vp, U32]),
         "ck_launch_digest": (C.c_int, [vp, P(Params), P(U32), U32, U32, P(U32), P(U32), P(U32), P(U64),
@@ -161,7 +162,7 @@ EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
            "ck_engine_release", "ck_engine_stats_get", "ck_trial_inputs_of", "ck_round_cache",
            "ck_ctx_set_placement", "ck_ctx_plan", "ck_plan_describe", "ck_launch_static",
-           "ck_ctx_set_place_search", "ck_ctx_place_report"]
+           "ck_ctx_set_place_search", "ck_ctx_place_report", "ck_ctx_place_regions"]
 
 
 def _check(rc):
@@ -257,16 +258,25 @@ class Context:
         _check(LIB.ck_ctx_set_layout(self.h, layout))
 
     def set_place_search(self, candidates: int):
-        """Table-region placement search: candidates per search (0 = off)."""
+        """Table placement on (candidates of the fallback search, 2..64) or off (0)."""
         _check(LIB.ck_ctx_set_place_search(self.h, candidates))
 
     def place_report(self):
         """The last placement search: (walk ms per candidate, kept indices)."""
-        ms = (C.c_float * 64)()
+        ms = (C.c_float * 256)()
         n = C.c_uint32()
         kept = (C.c_int32 * 2)()
-        _check(LIB.ck_ctx_place_report(self.h, ms, 64, C.byref(n), kept))
-        return [float(x) for x in ms[:min(n.value, 64)]], (kept[0], kept[1])
+        _check(LIB.ck_ctx_place_report(self.h, ms, 256, C.byref(n), kept))
+        return [float(x) for x in ms[:min(n.value, 256)]], (kept[0], kept[1])
+
+    def place_regions(self):
+        """The last region placement: (region of each piece, pieces of buffer 0,
+        pieces of buffer 1); ([], (-1,)*3, (-1,)*3) when none."""
+        reg = (C.c_int32 * 256)()
+        n = C.c_uint32()
+        used = (C.c_int32 * 6)()
+        _check(LIB.ck_ctx_place_regions(self.h, reg, 256, C.byref(n), used))
+        return [int(x) for x in reg[:min(n.value, 256)]], tuple(used[0:3]), tuple(used[3:6])
 
     def set_graph(self, on: bool):
         _check(LIB.ck_ctx_set_graph(self.h, 1 if on else 0))

@@ -1,6 +1,7 @@
-"""Table-region placement search: the library times a random walk over each
-candidate table region and keeps the two fastest as the tables of its pass
-buffers.  Results must not depend on it."""
+"""Table placement: the tables of a placed plan's pass buffers are held apart
+from them, spread over the device's memory regions (pieces sorted into regions
+by pair walks), or, where the pieces would be too small, the two fastest of
+candidate allocations.  Results must not depend on it."""
 import pytest
 
 from paths import load_product
@@ -10,30 +11,43 @@ pytestmark = pytest.mark.gpu
 ck = load_product()
 
 
-def test_place_search_keeps_the_two_fastest_and_changes_no_result():
-    p = ck.params("bgh25", 16384, 16, kick=500)  # 6-13 GB table regions: searched
+def _check_regions(ctx):
+    """The region report of a context whose tables were placed by region."""
+    ms, kept = ctx.place_report()
+    regions, b0, b1 = ctx.place_regions()
+    assert len(regions) >= 6 and set(regions) >= {0, 1} and regions[0] == 0
+    assert len(set(b0 + b1)) == 6 and min(b0 + b1) >= 0 and max(b0 + b1) < len(regions)
+    assert min(regions[i] for i in b0 + b1) >= 0  # no piece left out as unclear
+    for pieces in (b0, b1):
+        assert len({regions[i] for i in pieces}) >= 2  # every table range spans two regions or more
+    # The candidate table ranges were timed; the two fastest became the tables.
+    assert 2 <= len(ms) <= 6 and all(x > 0 for x in ms)
+    order = sorted(range(len(ms)), key=lambda i: ms[i])
+    assert (ms[kept[0]], ms[kept[1]]) == (ms[order[0]], ms[order[1]])
+    return regions, b0, b1
+
+
+def test_region_placement_spreads_tables_and_changes_no_result():
+    p = ck.params("bgh25", 16384, 16, kick=500)  # tables of 6.4 GB per pass buffer: pieces of 2.1 GB
     rk = ck.batch_key(1, 0)
     with ck.Context(0) as a:
         a.set_place_search(16)
         f1, s1 = a.launch(p, rk, 0, 1 << 17)
-        ms, kept = a.place_report()
-        assert len(ms) >= 3 and all(x > 0 for x in ms)
-        order = sorted(range(len(ms)), key=lambda i: ms[i])
-        assert (ms[kept[0]], ms[kept[1]]) == (ms[order[0]], ms[order[1]])
-        f1b, s1b = a.launch(p, rk, 0, 1 << 17)  # same buffers, no new search
-        assert a.place_report() == (ms, kept)
-    with ck.Context(0) as b:  # the default: no search
+        report = _check_regions(a)
+        f1b, s1b = a.launch(p, rk, 0, 1 << 17)  # same tables, no new placement
+        assert a.place_regions() == report
+    with ck.Context(0) as b:  # the default: no placement
         f2, s2 = b.launch(p, rk, 0, 1 << 17)
-        assert b.place_report()[1] == (-1, -1)
+        assert b.place_report()[1] == (-1, -1) and b.place_regions()[0] == []
     assert list(f1) == list(f2) == list(f1b)
     assert s1.probes == s2.probes == s1b.probes
 
 
 @pytest.mark.parametrize("layout", [ck.LAYOUT_DESC_PLAIN, ck.LAYOUT_RESIDUAL])
-def test_place_search_follows_the_table_size(layout):
-    """A plan with tables of another size searches again; every searched run
-    matches the unsearched one (u64 descriptor entries and u32 residual
-    entries, two shapes)."""
+def test_region_placement_follows_the_table_size(layout):
+    """A plan with tables of another size places them again; every placed run
+    matches the unplaced one (u64 descriptor entries and u32 residual entries,
+    two shapes)."""
     shapes = [ck.params("bgh25", 16384, 16, kick=500), ck.params("bgh25", 8192, 16, kick=500)]
     rk = ck.batch_key(1, 0)
     with ck.Context(0) as a, ck.Context(0) as b:
@@ -43,12 +57,30 @@ def test_place_search_follows_the_table_size(layout):
         reports = []
         for p in shapes:
             fa, sa = a.launch(p, rk, 0, 1 << 17)
-            reports.append(a.place_report())
+            reports.append(_check_regions(a))
             fb, sb = b.launch(p, rk, 0, 1 << 17)
             assert list(fa) == list(fb) and sa.probes == sb.probes
-        for ms, kept in reports:
-            assert 3 <= len(ms) <= 8 and -1 not in kept
         assert reports[0] != reports[1]
+
+
+def test_search_where_pieces_would_be_small():
+    """Tables of 1.6 GB per pass buffer (n = 4096) would split into pieces under
+    1 GiB: the library searches whole candidate allocations and keeps the two
+    fastest."""
+    p = ck.params("bgh25", 4096, 16, kick=500)
+    rk = ck.batch_key(3, 0)
+    with ck.Context(0) as a, ck.Context(0) as b:
+        a.set_place_search(8)
+        a.set_layout(ck.LAYOUT_RESIDUAL)
+        b.set_layout(ck.LAYOUT_RESIDUAL)
+        fa, sa = a.launch(p, rk, 0, 1 << 17)
+        ms, kept = a.place_report()
+        assert a.place_regions()[0] == []
+        assert 3 <= len(ms) <= 8 and all(x > 0 for x in ms)
+        order = sorted(range(len(ms)), key=lambda i: ms[i])
+        assert (ms[kept[0]], ms[kept[1]]) == (ms[order[0]], ms[order[1]])
+        fb, sb = b.launch(p, rk, 0, 1 << 17)
+        assert list(fa) == list(fb) and sa.probes == sb.probes
 
 
 def test_place_search_skips_small_and_graph_plans():
