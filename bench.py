@@ -196,20 +196,25 @@ def random_access_ceiling(probes_per_s: float, requests_per_probe):
     """Runs lib/ck_randbench: the chain's access pattern (each lane a dependent
     random read + write of one 8-byte slot of its own 192 KB table, with one
     XOR and one 32-bit multiply-high between a load and the next address)
-    without the chain's logic, at 32K..256K lanes and in 16 candidate buffers.
-    The ceiling is the best random-read request rate measured, with one walk
-    per lane or four independent walks per lane (latency out of the way)."""
+    without the chain's logic, at 32K..256K lanes, in 16 candidate buffers and
+    in one range spread over the device's memory regions (three pieces, as the
+    library places the chains' tables).  The ceiling is the best random-read
+    request rate measured, with one walk per lane or four independent walks per
+    lane (latency out of the way)."""
     import re
     import subprocess
     from paths import PKG
     exe = PKG / "lib" / "ck_randbench"
     if not exe.exists():
         return None
-    out = subprocess.run([str(exe), "--search", "16", "196608", "16384", "32768", "65536", "131072", "262144"],
-                         capture_output=True, text=True, timeout=300)
+    out = subprocess.run([str(exe), "--search", "16", "--regions", "196608", "16384", "32768", "65536", "131072",
+                          "262144"], capture_output=True, text=True, timeout=300)
     forms = r"(read_write|read_only_w4|read_only)"
     rows = re.findall(r"^" + forms + r" lanes=(\d+) .*steps_per_s=([0-9.e+]+)", out.stdout, re.M)
     found = re.findall(r"^search_" + forms + r" buffer=(\d+) .*steps_per_s=([0-9.e+]+)", out.stdout, re.M)
+    spread = {form: float(rate) for form, rate in
+              re.findall(r"^regions_" + forms + r" lanes=65536 .*steps_per_s=([0-9.e+]+)", out.stdout, re.M)}
+    nreg = re.findall(r"^regions_found=(\d+)", out.stdout, re.M)
     if out.returncode != 0 or not rows:
         return {"error": out.stderr[-300:]}
     sweep = {}
@@ -221,8 +226,9 @@ def random_access_ceiling(probes_per_s: float, requests_per_probe):
     best_lanes = max(sweep, key=lambda n: sweep[n].get("read_write", 0))
     # The best over the lane sweep and the 65,536-lane buffers of the search:
     # the chain's pass buffers are the fastest of its own search.
-    rw = max([sweep[best_lanes]["read_write"]] + [v.get("read_write", 0) for v in search.values()])
-    rows_all = list(sweep.values()) + list(search.values())
+    rw = max([sweep[best_lanes]["read_write"]] + [v.get("read_write", 0) for v in search.values()] +
+             [spread.get("read_write", 0)])
+    rows_all = list(sweep.values()) + list(search.values()) + [spread]
     ro1 = max(v.get("read_only", 0) for v in rows_all)
     ro4 = max(v.get("read_only_w4", 0) for v in rows_all)
     ro = max(ro1, ro4)
@@ -234,6 +240,9 @@ def random_access_ceiling(probes_per_s: float, requests_per_probe):
             "read_only_one_walk_steps_per_s": ro1, "read_only_four_walks_steps_per_s": ro4,
             "sweep": [{"lanes": n, **sweep[n]} for n in sorted(sweep)],
             "search_65536_lanes": [{"buffer": i, **search[i]} for i in sorted(search)],
+            # One range spread over the memory regions, as the library places
+            # the chains' tables.
+            "regions_65536_lanes": {"regions": int(nreg[0]) if nreg else None, **spread},
             "chain_probes_per_s": probes_per_s,
             "chain_over_read_write_steps": probes_per_s / rw,
             # The bound is the DRAM random-request rate: the chain's requests per

@@ -25,10 +25,17 @@
 //   search_read_write buffer=i lanes=65536 ... steps_per_s=R
 //   search_read_only  ...
 //   search_read_only_w4 ...
+// With --regions it then builds one range for 65,536 lanes out of three
+// pieces from the device's three memory regions (as the library places its
+// cuckoo tables) and prints
+//   regions_found=R piece_bytes=P
+//   regions_read_write lanes=65536 ... steps_per_s=R    (and the other forms)
 // bench.py takes the best read-only rate over all lines (either form) as the
 // ceiling that it sets the chain's DRAM request rate against.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -92,7 +99,167 @@ __global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64
         }                                                                                     \
     } while (0)
 
+// Two pieces, the first `half` lanes over tables of `slots` words at a, the
+// rest at b: the read + write walk, whose rate tells whether a and b lie in
+// one memory region.
+__global__ __launch_bounds__(256) void k_pair(uint64_t* __restrict__ a, uint64_t* __restrict__ b, uint32_t half,
+                                              uint32_t slots, uint32_t steps, unsigned long long* __restrict__ sink) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= 2 * half) return;
+    uint64_t* t = (lane < half ? a : b) + uint64_t(lane < half ? lane : lane - half) * slots;
+    uint32_t ctr = lane * kGolden32;
+    uint32_t i = __umulhi(static_cast<uint32_t>(mix64(lane)), slots);
+    uint64_t acc = 0;
+    for (uint32_t s = 0; s < steps; ++s) {
+        const uint64_t v = t[i];
+        t[i] = v ^ ctr;
+        acc += v;
+        i = __umulhi(static_cast<uint32_t>(v) ^ ctr, slots);
+        ctr += kGolden32;
+    }
+    if (acc == 0x5eedull) sink[0] = acc;
+}
+
 }  // namespace
+
+// Two modes among the times v: 2-means on their logarithms, each mode the
+// median of its cluster.  False unless both clusters hold two or more and the
+// slow mode is at least 1.12 times the fast one.
+static bool split_modes(std::vector<float> v, float* fast, float* slow) {
+    for (float& x : v) x = std::log(x);
+    if (v.size() < 4) return false;
+    std::sort(v.begin(), v.end());
+    float lo = v.front(), hi = v.back();
+    size_t cut = 0;
+    for (int it = 0; it < 32; ++it) {
+        cut = static_cast<size_t>(std::upper_bound(v.begin(), v.end(), 0.5f * (lo + hi)) - v.begin());
+        if (cut == 0 || cut == v.size()) return false;
+        float sl = 0, sh = 0;
+        for (size_t i = 0; i < v.size(); ++i) (i < cut ? sl : sh) += v[i];
+        const float nlo = sl / cut, nhi = sh / (v.size() - cut);
+        if (nlo == lo && nhi == hi) break;
+        lo = nlo;
+        hi = nhi;
+    }
+    if (cut < 2 || v.size() - cut < 2) return false;
+    *fast = std::exp(v[cut / 2]);
+    *slow = std::exp(v[cut + (v.size() - cut) / 2]);
+    return *slow >= *fast * 1.12f;
+}
+
+// A range of `bytes` made of three virtual-memory pieces from three memory
+// regions (two where only two show), found as the library's table placement
+// finds them: pieces allocated in batches, each paired with piece 0 and, if
+// apart from it, with the piece most clearly apart (pair walk with 24 KB
+// tables, 768 MiB of each piece), until every region holds one.
+struct Spread {
+    char* va = nullptr;
+    size_t piece = 0;
+    int regions = 0;
+    hipMemGenericAllocationHandle_t h[3] = {};
+};
+static int spread_range(size_t bytes, unsigned long long* sink, hipEvent_t a, hipEvent_t b, Spread* out) {
+    const size_t align = size_t(2) << 20;
+    const size_t P = ((bytes + 2) / 3 + align - 1) & ~(align - 1);
+    if (P < (size_t(1) << 30)) {
+        std::fprintf(stderr, "--regions needs pieces of 1 GiB or more\n");
+        return 2;
+    }
+    size_t free_b = 0, total_b = 0;
+    CHECK(hipMemGetInfo(&free_b, &total_b));
+    const size_t maxp = std::min<size_t>(96, free_b / 10 * 9 / P);
+    hipMemAllocationProp prop{};
+    prop.type = hipMemAllocationTypePinned;
+    prop.location.type = hipMemLocationTypeDevice;
+    prop.location.id = 0;
+    hipMemAccessDesc acc{};
+    acc.location = prop.location;
+    acc.flags = hipMemAccessFlagsProtReadWrite;
+    char* pool = nullptr;
+    CHECK(hipMemAddressReserve(reinterpret_cast<void**>(&pool), maxp * P, 0, nullptr, 0));
+    std::vector<hipMemGenericAllocationHandle_t> h;
+    std::vector<float> t0, t1;
+    std::vector<int> region;
+    size_t ref1 = 0;
+    auto pair = [&](size_t x, size_t y, float* ms) {
+        for (int rep = 0; rep < 2; ++rep) {
+            CHECK(hipEventRecord(a));
+            hipLaunchKernelGGL(k_pair, dim3(256), dim3(256), 0, 0, reinterpret_cast<uint64_t*>(pool + x * P),
+                               reinterpret_cast<uint64_t*>(pool + y * P), 32768u, 3072u, 1024u, sink);
+            CHECK(hipGetLastError());
+            CHECK(hipEventRecord(b));
+            CHECK(hipEventSynchronize(b));
+            CHECK(hipEventElapsedTime(ms, a, b));
+        }
+        return 0;
+    };
+    int pick[3] = {0, -1, -1};
+    while (h.size() < maxp && pick[2] < 0) {
+        const size_t from = h.size();
+        for (int k = 0; k < 4 && h.size() < maxp; ++k) {
+            hipMemGenericAllocationHandle_t x{};
+            if (hipMemCreate(&x, P, &prop, 0) != hipSuccess) break;
+            CHECK(hipMemMap(pool + h.size() * P, P, 0, x, 0));
+            CHECK(hipMemSetAccess(pool + h.size() * P, P, &acc, 1));
+            h.push_back(x);
+        }
+        if (h.size() == from) break;
+        t0.resize(h.size(), 0.f);
+        t1.resize(h.size(), 0.f);
+        region.resize(h.size(), 0);
+        for (size_t j = std::max<size_t>(from, 1); j < h.size(); ++j)
+            if (pair(0, j, &t0[j])) return 1;
+        float f0 = 0, s0 = 0, f1 = 0, s1 = 0;
+        if (!split_modes(std::vector<float>(t0.begin() + 1, t0.end()), &f0, &s0)) continue;
+        std::vector<size_t> others;
+        for (size_t j = 1; j < h.size(); ++j) {
+            region[j] = t0[j] >= s0 / 1.06f ? 0 : t0[j] <= f0 * 1.06f ? 1 : -1;
+            if (region[j] == 1) others.push_back(j);
+        }
+        if (!ref1 && !others.empty()) {
+            ref1 = others[0];
+            for (size_t j : others)
+                if (t0[j] < t0[ref1]) ref1 = j;
+        }
+        std::vector<float> v1;
+        for (size_t j : others)
+            if (j != ref1) {
+                if (t1[j] == 0 && pair(ref1, j, &t1[j])) return 1;
+                v1.push_back(t1[j]);
+            }
+        if (split_modes(v1, &f1, &s1))
+            for (size_t j : others)
+                if (j != ref1) region[j] = t1[j] >= s1 / 1.06f ? 1 : t1[j] <= f1 * 1.06f ? 2 : -1;
+        pick[1] = pick[2] = -1;
+        for (size_t j = 1; j < h.size(); ++j)
+            if (region[j] > 0 && pick[region[j]] < 0) pick[region[j]] = int(j);
+    }
+    out->regions = pick[1] < 0 ? 1 : pick[2] < 0 ? 2 : 3;
+    if (pick[1] < 0) pick[1] = 1;  // one region seen: any pieces
+    if (pick[2] < 0) {              // two: the second piece of the first region
+        pick[2] = -1;
+        for (size_t j = 1; j < h.size() && pick[2] < 0; ++j)
+            if (region[j] == 0) pick[2] = int(j);
+        if (pick[2] < 0) pick[2] = 2;
+    }
+    if (h.size() < 3) {
+        std::fprintf(stderr, "--regions: fewer than three pieces\n");
+        return 2;
+    }
+    CHECK(hipMemAddressReserve(reinterpret_cast<void**>(&out->va), 3 * P, 0, nullptr, 0));
+    for (int r = 0; r < 3; ++r) {
+        CHECK(hipMemMap(out->va + r * P, P, 0, h[pick[r]], 0));
+        out->h[r] = h[pick[r]];
+    }
+    CHECK(hipMemSetAccess(out->va, 3 * P, &acc, 1));
+    out->piece = P;
+    CHECK(hipDeviceSynchronize());
+    for (size_t j = 0; j < h.size(); ++j) CHECK(hipMemUnmap(pool + j * P, P));
+    CHECK(hipMemAddressFree(pool, maxp * P));
+    for (size_t j = 0; j < h.size(); ++j)
+        if (int(j) != pick[0] && int(j) != pick[1] && int(j) != pick[2]) CHECK(hipMemRelease(h[j]));
+    return 0;
+}
 
 // The three forms at `lanes` lanes over buf; prints one line per form with `tag`.
 static int measure(const char* tag, int buffer, uint64_t* buf, uint64_t pitch, uint32_t slots, uint32_t lanes,
@@ -130,9 +297,17 @@ static int measure(const char* tag, int buffer, uint64_t* buf, uint64_t pitch, u
 int main(int argc, char** argv) {
     int arg = 1;
     uint32_t search = 0;
-    if (argc > 2 && std::string(argv[1]) == "--search") {
-        search = uint32_t(std::strtoul(argv[2], nullptr, 10));
-        arg = 3;
+    bool regions = false;
+    for (;;) {
+        if (argc > arg + 1 && std::string(argv[arg]) == "--search") {
+            search = uint32_t(std::strtoul(argv[arg + 1], nullptr, 10));
+            arg += 2;
+        } else if (argc > arg && std::string(argv[arg]) == "--regions") {
+            regions = true;
+            arg += 1;
+        } else {
+            break;
+        }
     }
     const uint64_t table = argc > arg ? std::strtoull(argv[arg], nullptr, 10) : 196608ull;
     const uint32_t steps = argc > arg + 1 ? uint32_t(std::strtoul(argv[arg + 1], nullptr, 10)) : 16384u;
@@ -144,7 +319,8 @@ int main(int argc, char** argv) {
     if (table < 8 || (table & 7) || table / 8 > 0xffffffffull || steps == 0 || max_lanes == 0 ||
         max_lanes > (1u << 22) || search > 64) {
         std::fprintf(stderr,
-                     "usage: ck_randbench [--search K <= 64] [table_bytes, multiple of 8] [steps] [lanes <= 2^22 ...]\n");
+                     "usage: ck_randbench [--search K <= 64] [--regions] [table_bytes, multiple of 8] [steps] "
+                     "[lanes <= 2^22 ...]\n");
         return 2;
     }
     const uint32_t slots = static_cast<uint32_t>(table / 8);
@@ -190,6 +366,21 @@ int main(int argc, char** argv) {
             if (measure("search_", int(i), cand[i], pitch, slots, lanes, table, steps, sink, a, b)) return 1;
         }
         for (uint64_t* c : cand) CHECK(hipFree(c));
+    }
+    if (regions) {
+        const uint32_t lanes = 65536;
+        Spread sp;
+        if (int rc = spread_range(pitch * lanes * 8, sink, a, b, &sp)) return rc;
+        std::printf("regions_found=%d piece_bytes=%zu\n", sp.regions, sp.piece);
+        hipLaunchKernelGGL(k_fill, dim3(8192), dim3(256), 0, 0, reinterpret_cast<uint64_t*>(sp.va), pitch * lanes,
+                           0x243F6A8885A308D3ull);
+        CHECK(hipGetLastError());
+        if (measure("regions_", -1, reinterpret_cast<uint64_t*>(sp.va), pitch, slots, lanes, table, steps, sink, a, b))
+            return 1;
+        CHECK(hipDeviceSynchronize());
+        for (int r = 0; r < 3; ++r) CHECK(hipMemUnmap(sp.va + r * sp.piece, sp.piece));
+        for (int r = 0; r < 3; ++r) CHECK(hipMemRelease(sp.h[r]));
+        CHECK(hipMemAddressFree(sp.va, 3 * sp.piece));
     }
     CHECK(hipFree(sink));
     return 0;
