@@ -57,16 +57,16 @@ ALGO_BUCKET_INIT_BYTES = 4
 # stream), and the profile-derived HBM bytes and DRAM requests
 # (TCC_EA0_RDREQ + TCC_EA0_WRREQ) per probe from rocprofv3 --pmc.  The chain's
 # reads are 64-byte requests, which FETCH_SIZE counts exactly.
-# The figures are those of fast pass buffers (the ones the placement search
-# keeps): in slow buffers a probe reads 65.3 / 64.5 B and makes 1.876 / 1.864
-# requests (residual / descriptor).
+# The figures are those of tables placed over the three memory regions (as the
+# headline context places them); in one region a probe makes 1.876 / 1.864
+# requests (residual / descriptor), in a two-region straddle 1.836 / 1.833.
 CHAIN_PROFILES = {
     "desc-plain": dict(kernel="k_chain<3, bgh, u64 descriptor stream, u64 entries>", slot_bytes=16, item_bytes=8,
-                       read=62.46, write=27.44, requests=0.9758 + 0.8570,
-                       profile="profiles/k_chain_hbm_n16384_placement_search.md"),
+                       read=64.60, write=27.46, requests=1.0092 + 0.8577,
+                       profile="profiles/k_chain_hbm_n16384_region_placement.md"),
     "residual": dict(kernel="k_chain_res<3, bgh, u64 candidate stream, u32 entries>", slot_bytes=8, item_bytes=8,
-                     read=62.69, write=27.39, requests=0.9809 + 0.8550,
-                     profile="profiles/k_chain_hbm_n16384_placement_search.md"),
+                     read=65.08, write=27.42, requests=1.0170 + 0.8560,
+                     profile="profiles/k_chain_hbm_n16384_region_placement.md"),
 }
 # The n = 2^20 chain (44-bit residual entries, windowed): HBM bytes and DRAM
 # requests per probe and the L2 hit rate, from its counter profile.
