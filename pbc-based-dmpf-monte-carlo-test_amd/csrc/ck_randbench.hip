@@ -18,6 +18,8 @@
 //   read_write lanes=L table_bytes=B steps=S ms=T steps_per_s=R
 //   read_only  ...
 //   read_only_w4 ...   (four independent walks per lane, S steps in all)
+//   read_write_w4 ...  (the same with the write-back)
+//   read_write_7of8 ... (one walk per lane, seven steps in eight write back)
 // The rate belongs to the memory the tables land in (the chain's passes run
 // 15 % faster in some allocations than in others), so with --search K it then
 // allocates K buffers for 65,536 lanes, all held at once, runs both forms in
@@ -59,10 +61,12 @@ __global__ __launch_bounds__(256) void k_fill(uint64_t* __restrict__ buf, uint64
         buf[i] = mix64(i + salt);
 }
 
-// W independent walks per lane over the lane's table, steps / W steps each
+// W independent walks per lane over the lane's table, steps / W steps each;
+// WRITE 0: read only, 1: every step writes its slot back changed, 2: seven
+// steps in eight do
 // (W = 1: the chain's single dependent path; W = 4: four loads in flight per
 // lane, the read-only rate with latency taken out of the way).
-template <bool WRITE, int W>
+template <int WRITE, int W>
 __global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64_t pitch, uint32_t slots,
                                               uint32_t lanes, uint32_t steps, unsigned long long* __restrict__ sink) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
@@ -81,7 +85,9 @@ __global__ __launch_bounds__(256) void k_walk(uint64_t* __restrict__ buf, uint64
         for (int w = 0; w < W; ++w) v[w] = t[i[w]];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            if constexpr (WRITE) t[i[w]] = v[w] ^ ctr[w];
+            if constexpr (WRITE == 1) t[i[w]] = v[w] ^ ctr[w];
+            if constexpr (WRITE == 2)  // seven steps in eight write back (the chain writes back 0.86 per probe)
+                if (v[w] & 7) t[i[w]] = v[w] ^ ctr[w];
             acc += v[w];
             i[w] = __umulhi(static_cast<uint32_t>(v[w]) ^ ctr[w], slots);
             ctr[w] += kGolden32;
@@ -261,27 +267,32 @@ static int spread_range(size_t bytes, unsigned long long* sink, hipEvent_t a, hi
     return 0;
 }
 
-// The three forms at `lanes` lanes over buf; prints one line per form with `tag`.
+// The five forms at `lanes` lanes over buf; prints one line per form with `tag`.
 static int measure(const char* tag, int buffer, uint64_t* buf, uint64_t pitch, uint32_t slots, uint32_t lanes,
                    uint64_t table, uint32_t steps, unsigned long long* sink, hipEvent_t a, hipEvent_t b) {
     const dim3 grid((lanes + 255) / 256), block(256);
-    static const char* const kForm[3] = {"read_write", "read_only", "read_only_w4"};
-    for (int form = 0; form < 3; ++form) {
+    static const char* const kForm[5] = {"read_write", "read_only", "read_only_w4", "read_write_w4",
+                                         "read_write_7of8"};
+    for (int form = 0; form < 5; ++form) {
         float ms = 0;
         for (int rep = 0; rep < 2; ++rep) {  // the first run warms the TLB and clocks
             CHECK(hipEventRecord(a));
             if (form == 0)
-                hipLaunchKernelGGL((k_walk<true, 1>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+                hipLaunchKernelGGL((k_walk<1, 1>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
             else if (form == 1)
-                hipLaunchKernelGGL((k_walk<false, 1>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+                hipLaunchKernelGGL((k_walk<0, 1>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+            else if (form == 2)
+                hipLaunchKernelGGL((k_walk<0, 4>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+            else if (form == 3)
+                hipLaunchKernelGGL((k_walk<1, 4>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
             else
-                hipLaunchKernelGGL((k_walk<false, 4>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
+                hipLaunchKernelGGL((k_walk<2, 1>), grid, block, 0, 0, buf, pitch, slots, lanes, steps, sink);
             CHECK(hipGetLastError());
             CHECK(hipEventRecord(b));
             CHECK(hipEventSynchronize(b));
             CHECK(hipEventElapsedTime(&ms, a, b));
         }
-        const uint32_t done = form == 2 ? steps / 4 * 4 : steps;
+        const uint32_t done = form == 2 || form == 3 ? steps / 4 * 4 : steps;
         const double rate = double(lanes) * done / (ms / 1e3);
         if (buffer >= 0)
             std::printf("%s%s buffer=%d lanes=%u table_bytes=%llu steps=%u ms=%.3f steps_per_s=%.6e\n", tag,
