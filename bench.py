@@ -74,6 +74,9 @@ N20_PROFILE = dict(kernel="k_chain_res<3, bgh, u64 candidate stream, 44-bit entr
                    requests=0.897 + 0.885, l2_hit_rate=0.686, profile="profiles/k_chain_res_hbm_n2_20_counters.md")
 # DRAM requests per step of ck_randbench (profiles/randbench_192k_traffic.md).
 RB_READ_WRITE_REQ_PER_STEP = 0.998 + 1.0005
+# Engines of the sweep and serving legs take their arenas spread over the
+# device's memory regions (--no-spread: one allocation).
+SPREAD_ARENA = True
 RB_READ_ONLY_REQ_PER_STEP = 0.998
 RB_READ_ONLY_W4_REQ_PER_STEP = 0.9636  # four walks share a lane's table: a few more L2 hits
 RB_READ_WRITE_W4_REQ_PER_STEP = 0.9989 + 0.9998
@@ -336,7 +339,8 @@ def sweep_run(ck, pts, trials_of, plain_too, label):
     if plain_too:
         plain, plain_s, busy_s, plain_probes = plain_loop(ck, pts, ps, trials_of)
         log(f"{label}: per-point launches {plain_s:.2f} s (device busy {busy_s:.2f} s)")
-    eng = ck.Engine(0, policy=ck.SCHED_SRPT)
+    eng = ck.Engine(0, policy=ck.SCHED_SRPT, spread=SPREAD_ARENA)
+    regions = eng.arena_regions()
     t0 = time.perf_counter()
     ids = [eng.submit(p, pt["seed"], trials_of(pt), bt=pt["bt"]) for pt, p in zip(pts, ps)]
     res = wait_all(eng, ids, label, t0)
@@ -361,6 +365,8 @@ def sweep_run(ck, pts, trials_of, plain_too, label):
         "failed": sum(r.failed for r in res), "probes": probes,
         "probes_per_s": probes / eng_s, "plain_probes_per_s": plain_probes / plain_s if plain_s else None,
         "results_equal_plain": equal,
+        # The engine's arena: 1 GiB pieces per memory region (0, 1, 2, unclear).
+        "arena_regions": list(regions),
     }, equal
 
 
@@ -404,7 +410,7 @@ def serve_leg(ck, rates, max_jobs: int, budget_lo: float, budget_hi: float, plai
         jobs = max(1, min(max_jobs, max(200, int(rate * window_s))))
         shapes, ps = shapes_all[:jobs], ps_all[:jobs]
         arr = sw.arrivals(jobs, rate, seed=3, budget_lo=budget_lo, budget_hi=budget_hi)
-        eng = ck.Engine(0, policy=ck.SCHED_SRPT)
+        eng = ck.Engine(0, policy=ck.SCHED_SRPT, spread=SPREAD_ARENA)
         t0 = time.perf_counter()
         ids = []
         for (at, budget), pt, p in zip(arr, shapes, ps):
@@ -454,7 +460,7 @@ def serve_stated_leg(ck, rate: float, window_s: float, budget_lo: float = 1e5, b
     n = int(rate * window_s * 1.5) + 16
     shapes = sw.sweep_points(n, seed=3)
     arr = [a for a in sw.arrivals(n, rate, seed=3, budget_lo=budget_lo, budget_hi=budget_hi) if a[0] < window_s]
-    eng = ck.Engine(0, policy=ck.SCHED_SRPT)
+    eng = ck.Engine(0, policy=ck.SCHED_SRPT, spread=SPREAD_ARENA)
     t0 = time.perf_counter()
     ids, done = [], {}
     for (at, budget), pt in zip(arr, shapes):
@@ -669,11 +675,15 @@ def main():
                     help="candidate table regions the headline context times before keeping two (0: no search)")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-mode comparison")
     ap.add_argument("--no-tiers", action="store_true", help="skip the lane-tier comparison")
+    ap.add_argument("--no-spread", action="store_true",
+                    help="engines take their arena as one allocation, not spread over the memory regions")
     ap.add_argument("--tune-sample", type=int, default=1 << 17, help="trials per autotuner candidate")
     ap.add_argument("--hbm-only", action="store_true", help="run only the n=2^20 leg (for counter profiles)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before warmup, copy a buffer of this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
+    global SPREAD_ARENA
+    SPREAD_ARENA = not args.no_spread
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path "
               "(trial batches would shard across GPUs by batch id)", file=sys.stderr)

@@ -339,7 +339,10 @@ typedef struct ck_engine_config {
     uint32_t max_jobs;    /* jobs admitted at once: device counter slots (0: 4096) */
     uint32_t policy;      /* CK_SCHED_* */
     uint32_t window;      /* jobs considered per pass, in arrival order (0: all) */
-    uint32_t pad;
+    uint32_t spread;      /* 1: the arena is 1 GiB pieces taken from the device's memory regions in
+                             turn (random read + write traffic spread over the regions runs faster
+                             than traffic in one; see ck_ctx_set_place_search), or one allocation
+                             where the regions cannot be told apart; 0: one allocation */
 } ck_engine_config;
 
 typedef struct ck_engine_stats {
@@ -371,6 +374,10 @@ int ck_engine_failures(ck_engine* e, uint64_t job_id, uint64_t* out, uint64_t ca
 /* Drops a finished job's host record. */
 int ck_engine_release(ck_engine* e, uint64_t job_id);
 int ck_engine_stats_get(ck_engine* e, ck_engine_stats* out);
+/* The arena's 1 GiB pieces per memory region: counts[0..2] for the three
+ * regions, counts[3] for pieces whose region was unclear; all zero when the
+ * arena is one allocation.  No reference counterpart. */
+int ck_engine_arena(ck_engine* e, uint32_t counts[4]);
 
 #ifdef __cplusplus
 }

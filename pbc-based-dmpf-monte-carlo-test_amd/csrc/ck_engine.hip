@@ -40,6 +40,7 @@
 #include "ck_chain.hpp"
 #include "ck_device.hpp"
 #include "ck_internal.hpp"
+#include "ck_regions.hpp"
 
 namespace ck {
 namespace eng {
@@ -236,6 +237,11 @@ struct PassBuf {
     bool keep = false;
 };
 
+// Spread arenas (ck_engine_config.spread): pieces of 1 GiB, at most 256 held
+// while they are sorted into regions.
+static constexpr size_t kArenaPiece = size_t(1) << 30;
+static constexpr size_t kArenaMaxPieces = 256;
+
 struct Engine {
     int device = 0;
     ck_engine_config cfg{};
@@ -244,6 +250,11 @@ struct Engine {
     std::vector<JobCounters> jc_h;  // host copy of the counter slots
     char* arena_base = nullptr;
     size_t arena_per = 0;
+    // A spread arena (cfg.spread): kArenaPiece pieces mapped at arena_base,
+    // from the memory regions in turn; arena_count[r] pieces of region r
+    // (arena_count[3]: unclear).  Empty: one hipMalloc.
+    std::vector<hipMemGenericAllocationHandle_t> arena_pieces;
+    uint32_t arena_count[4] = {0, 0, 0, 0};
     uint32_t* te0_d = nullptr;
     JobCounters* jc_d = nullptr;
     size_t lds_block = 64 * 1024;
@@ -637,6 +648,45 @@ static void driver_main(Engine* ep) {
     E.cv_done.notify_all();
 }
 
+// A spread arena: whole kArenaPiece pieces (the arena rounded down to them),
+// sorted into the device's memory regions by pair walks and mapped in the
+// regions' turn, so that every pass buffer's maps and tables are spread over
+// the regions; the pass buffers keep their offsets.  Leaves arena_base null
+// (one hipMalloc follows) where fewer than two regions show.
+static int spread_arena(Engine& E) {
+    const size_t n = E.arena_per * E.cfg.buffers / kArenaPiece;
+    size_t free_b = 0, total_b = 0;
+    EH(hipMemGetInfo(&free_b, &total_b));
+    const size_t max_pieces = std::min<size_t>(kArenaMaxPieces, free_b / 20 * 19 / kArenaPiece);
+    if (max_pieces < n) return CK_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    unsigned long long* sink = nullptr;
+    EH(hipMalloc(&sink, 256));
+    EH(hipEventCreate(&e0));
+    EH(hipEventCreate(&e1));
+    RegionPool rp;
+    std::string err;
+    int rc = region_pool(E.device, E.cstream[0], e0, e1, sink, kArenaPiece, max_pieces, (n + 2) / 3, &rp, &err);
+    if (rc == CK_OK && rp.two) {
+        const std::vector<size_t> order = interleave(rp, n);
+        void* va = nullptr;
+        if (order.size() == n && (rc = map_pieces(E.device, rp, order, &va, &err)) == CK_OK) {
+            E.arena_base = static_cast<char*>(va);
+            for (size_t j : order) {
+                E.arena_pieces.push_back(rp.h[j]);
+                ++E.arena_count[rp.region[j] < 0 ? 3 : rp.region[j]];
+            }
+            E.arena_per = n * kArenaPiece / E.cfg.buffers & ~size_t(255);
+        }
+    }
+    drop_pool(rp, E.arena_pieces);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(sink);
+    if (rc) return set_fail(E, "spread arena: " + err);
+    return CK_OK;
+}
+
 static int engine_init(Engine& E) {
     EH(hipSetDevice(E.device));
     EH(hipStreamCreateWithFlags(&E.bstream, hipStreamNonBlocking));
@@ -664,7 +714,10 @@ static int engine_init(Engine& E) {
         E.cfg.arena_bytes = std::min<uint64_t>(uint64_t(192) << 30, free_b / 6 * 5);
     }
     E.arena_per = (E.cfg.arena_bytes / E.cfg.buffers) & ~size_t(255);
-    EH(hipMalloc(&E.arena_base, E.arena_per * E.cfg.buffers));
+    if (E.cfg.spread && E.arena_per * E.cfg.buffers >= 3 * kArenaPiece) {
+        if (int rc = spread_arena(E)) return rc;
+    }
+    if (!E.arena_base) EH(hipMalloc(&E.arena_base, E.arena_per * E.cfg.buffers));
     const uint32_t maxseg = E.cfg.max_slots / SEG;
     E.bufs.resize(E.cfg.buffers);
     for (uint32_t b = 0; b < E.cfg.buffers; ++b) {
@@ -698,7 +751,13 @@ static void engine_free(Engine& E) {
         for (hipEvent_t ev : {B.eb0, B.eb1, B.ec0, B.ec1})
             if (ev) (void)hipEventDestroy(ev);
     }
-    if (E.arena_base) (void)hipFree(E.arena_base);
+    if (E.arena_base && !E.arena_pieces.empty()) {
+        (void)hipDeviceSynchronize();
+        unmap_pieces(E.arena_base, kArenaPiece, E.arena_pieces.size());
+        for (hipMemGenericAllocationHandle_t h : E.arena_pieces) (void)hipMemRelease(h);
+    } else if (E.arena_base) {
+        (void)hipFree(E.arena_base);
+    }
     if (E.jc_d) (void)hipFree(E.jc_d);
     if (E.te0_d) (void)hipFree(E.te0_d);
     if (E.bstream) (void)hipStreamDestroy(E.bstream);
@@ -732,6 +791,7 @@ int ck_engine_create(int device, const ck_engine_config* cfg, ck_engine** out) {
     if (c.max_slots % SEG || c.max_slots > (1u << 22)) return fail(CK_EINVAL, "max_slots must be a multiple of 64, <= 2^22");
     if (c.policy > CK_SCHED_SRPT) return fail(CK_EINVAL, "bad scheduling policy");
     if (c.max_jobs > (1u << 20)) return fail(CK_EINVAL, "max_jobs too large");
+    if (c.spread > 1) return fail(CK_EINVAL, "spread must be 0 or 1");
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return fail(CK_EINVAL, "no such HIP device");
     auto* h = new ck_engine();
@@ -877,6 +937,12 @@ int ck_engine_stats_get(ck_engine* h, ck_engine_stats* out) {
     std::lock_guard<std::mutex> lk(E.mu);
     *out = E.st;
     out->wall_ms = E.now_ms();
+    return CK_OK;
+}
+
+int ck_engine_arena(ck_engine* h, uint32_t counts[4]) {
+    if (!h || !counts) return fail(CK_EINVAL, "null argument");
+    for (int i = 0; i < 4; ++i) counts[i] = h->e.arena_count[i];
     return CK_OK;
 }
 

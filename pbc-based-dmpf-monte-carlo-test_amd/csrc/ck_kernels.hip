@@ -31,6 +31,7 @@
 #include "ck_device.hpp"
 #include "ck_internal.hpp"
 #include "ck_lane.hpp"
+#include "ck_regions.hpp"
 
 namespace ck {
 
@@ -1606,19 +1607,22 @@ static bool placeable(const Ctx& c, const HbmDecision& d) {
            c.budget >= 2 * d.h.off_tables + 3 * (d.h.buffer - d.h.off_tables);
 }
 
-// Event-timed launch of `launch` on the chain stream: once to warm the
-// translations, once timed.
+// Event-timed launch of `launch` on stream s: once to warm the translations,
+// once timed.
 template <typename F>
-static int timed_twice(Ctx& c, F&& launch, float* ms) {
+static hipError_t timed_twice(hipStream_t s, hipEvent_t e0, hipEvent_t e1, F&& launch, float* ms) {
     for (int rep = 0; rep < 2; ++rep) {
-        CKH(hipEventRecord(c.ev_chain[0][0], c.stream));
-        launch();
-        CKH(hipGetLastError());
-        CKH(hipEventRecord(c.ev_chain[0][1], c.stream));
-        CKH(hipEventSynchronize(c.ev_chain[0][1]));
-        CKH(hipEventElapsedTime(ms, c.ev_chain[0][0], c.ev_chain[0][1]));
+        hipError_t e = hipEventRecord(e0, s);
+        if (e == hipSuccess) {
+            launch();
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+        if (e != hipSuccess) return e;
     }
-    return CK_OK;
+    return hipSuccess;
 }
 
 // Two modes among the times t[idx]: 2-means on their logarithms (so a few
@@ -1651,14 +1655,185 @@ static bool split_modes(const std::vector<float>& t, const std::vector<size_t>& 
     return *slow >= *fast * kRegionModes;
 }
 
-// Region placement (see above).  *placed stays false, with nothing allocated
-// beyond ws, when it does not apply: pieces too small, no virtual-memory
-// support, fewer than two regions seen, or too few pieces in them.  Pieces
-// are allocated in batches and sorted as they come, and allocation stops once
-// every region holds kRegionCands of them: a device whose memory was freed in
-// scattered pieces shows all three regions within a few batches (and makes
-// each new piece wait for the driver to clear it), a fresh one only after the
-// first two regions are used up.
+static void region_access(int device, hipMemAllocationProp* prop, hipMemAccessDesc* acc) {
+    *prop = hipMemAllocationProp{};
+    prop->type = hipMemAllocationTypePinned;
+    prop->location.type = hipMemLocationTypeDevice;
+    prop->location.id = device;
+    *acc = hipMemAccessDesc{};
+    acc->location = prop->location;
+    acc->flags = hipMemAccessFlagsProtReadWrite;
+}
+
+// Pieces are allocated in batches and sorted as they come, and allocation
+// stops once every region holds `want` of them: a device whose memory was
+// freed in scattered pieces shows all three regions within a few batches (and
+// makes each new piece wait for the driver to clear it), a fresh one only
+// after the first two regions are used up.  t0: each piece paired with piece
+// 0; t1: each piece of another region than piece 0's paired with ref1, the one
+// most clearly apart from piece 0.  Pieces near the slow mode of t0 share
+// piece 0's region; near the fast one they do not, and a second split among
+// them in t1, where it shows, is a third region; pieces in between are left
+// out (-1).  (Where the memory after the first region was handed out in
+// pieces that each span two regions, no second split shows.)
+int region_pool(int device, hipStream_t stream, hipEvent_t e0, hipEvent_t e1, unsigned long long* sink,
+                size_t piece, size_t max_pieces, size_t want, RegionPool* out, std::string* err) {
+    RegionPool& R = *out;
+    R = RegionPool{};
+    R.piece = piece;
+    if (piece < kRegionMinPiece || piece % kRegionPieceAlign || max_pieces < 2) return CK_OK;
+    hipMemAllocationProp prop;
+    hipMemAccessDesc acc;
+    region_access(device, &prop, &acc);
+    if (hipMemAddressReserve(reinterpret_cast<void**>(&R.pool), max_pieces * piece, 0, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        R.pool = nullptr;
+        return CK_OK;
+    }
+    R.reserved = max_pieces * piece;
+    auto pair_ms = [&](size_t a, size_t b, float* ms) {
+        return timed_twice(
+            stream, e0, e1,
+            [&]() {
+                hipLaunchKernelGGL(k_place_pair, dim3(kRegionPairLanes / 256), dim3(256), 0, stream,
+                                   reinterpret_cast<uint64_t*>(R.pool + a * piece),
+                                   reinterpret_cast<uint64_t*>(R.pool + b * piece), kRegionPairLanes / 2,
+                                   kRegionPairSlots, kRegionPairSteps, sink);
+            },
+            ms);
+    };
+    std::vector<float> t0, t1;
+    size_t ref1 = 0;
+    bool full = false;
+    while (!full) {
+        const size_t from = R.ready;
+        for (size_t k = 0; k < kRegionBatch && R.ready < max_pieces; ++k) {
+            hipMemGenericAllocationHandle_t x{};
+            if (hipMemCreate(&x, piece, &prop, 0) != hipSuccess) {  // the device is full (or has no such API)
+                (void)hipGetLastError();
+                full = true;
+                break;
+            }
+            R.h.push_back(x);
+            hipError_t e = hipMemMap(R.pool + R.ready * piece, piece, 0, x, 0);
+            if (e == hipSuccess) {
+                ++R.mapped;
+                e = hipMemSetAccess(R.pool + R.ready * piece, piece, &acc, 1);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                full = true;
+                break;
+            }
+            ++R.ready;
+        }
+        if (R.ready == max_pieces) full = true;
+        if (R.ready == from) break;
+        const size_t n = R.ready;
+        t0.resize(n, 0.f);
+        t1.resize(n, 0.f);
+        R.region.resize(n, 0);
+        for (size_t j = std::max<size_t>(from, 1); j < n; ++j)
+            if (hipError_t e = pair_ms(0, j, &t0[j])) {
+                *err = std::string("pair walk: ") + hipGetErrorString(e);
+                return CK_EDEVICE;
+            }
+        std::vector<size_t> all;
+        for (size_t j = 1; j < n; ++j) all.push_back(j);
+        float f0 = 0, s0 = 0;
+        R.two = split_modes(t0, all, &f0, &s0);
+        std::vector<size_t> others;
+        for (size_t j = 1; j < n; ++j) {
+            R.region[j] = !R.two ? 0 : t0[j] >= s0 / kRegionNear ? 0 : t0[j] <= f0 * kRegionNear ? 1 : -1;
+            if (R.region[j] == 1) others.push_back(j);
+        }
+        if (R.two && !ref1 && !others.empty()) {
+            ref1 = others[0];
+            for (size_t j : others)
+                if (t0[j] < t0[ref1]) ref1 = j;
+        }
+        std::vector<size_t> others1;
+        for (size_t j : others)
+            if (j != ref1) {
+                if (t1[j] == 0)
+                    if (hipError_t e = pair_ms(ref1, j, &t1[j])) {
+                        *err = std::string("pair walk: ") + hipGetErrorString(e);
+                        return CK_EDEVICE;
+                    }
+                others1.push_back(j);
+            }
+        float f1 = 0, s1 = 0;
+        if (ref1 && split_modes(t1, others1, &f1, &s1))
+            for (size_t j : others1) R.region[j] = t1[j] >= s1 / kRegionNear ? 1 : t1[j] <= f1 * kRegionNear ? 2 : -1;
+        if (R.of(0).size() >= want && R.of(1).size() >= want && R.of(2).size() >= want) break;
+    }
+    return CK_OK;
+}
+
+int map_pieces(int device, const RegionPool& p, const std::vector<size_t>& pick, void** va, std::string* err) {
+    *va = nullptr;
+    hipMemAllocationProp prop;
+    hipMemAccessDesc acc;
+    region_access(device, &prop, &acc);
+    const size_t bytes = pick.size() * p.piece;
+    if (hipMemAddressReserve(va, bytes, 0, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        *va = nullptr;
+        *err = "hipMemAddressReserve of " + std::to_string(bytes >> 20) + " MiB";
+        return CK_EDEVICE;
+    }
+    size_t n = 0;
+    while (n < pick.size() && hipMemMap(static_cast<char*>(*va) + n * p.piece, p.piece, 0, p.h[pick[n]], 0) == hipSuccess)
+        ++n;
+    if (n == pick.size() && hipMemSetAccess(*va, bytes, &acc, 1) == hipSuccess) return CK_OK;
+    (void)hipGetLastError();
+    for (size_t i = 0; i < n; ++i) (void)hipMemUnmap(static_cast<char*>(*va) + i * p.piece, p.piece);
+    (void)hipMemAddressFree(*va, bytes);
+    *va = nullptr;
+    *err = "mapping pieces into a range";
+    return CK_EDEVICE;
+}
+
+void unmap_pieces(void* va, size_t piece, size_t n) {
+    if (!va) return;
+    for (size_t i = 0; i < n; ++i) (void)hipMemUnmap(static_cast<char*>(va) + i * piece, piece);
+    (void)hipMemAddressFree(va, n * piece);
+}
+
+void drop_pool(RegionPool& p, const std::vector<hipMemGenericAllocationHandle_t>& keep) {
+    (void)hipDeviceSynchronize();
+    for (size_t j = 0; j < p.mapped; ++j) (void)hipMemUnmap(p.pool + j * p.piece, p.piece);
+    if (p.pool) (void)hipMemAddressFree(p.pool, p.reserved);
+    for (hipMemGenericAllocationHandle_t x : p.h)
+        if (std::find(keep.begin(), keep.end(), x) == keep.end()) (void)hipMemRelease(x);
+    p = RegionPool{};
+}
+
+std::vector<size_t> interleave(const RegionPool& p, size_t n) {
+    const std::vector<size_t> q[3] = {p.of(0), p.of(1), p.of(2)};
+    size_t at[3] = {0, 0, 0};
+    std::vector<size_t> out;
+    for (bool any = true; any && out.size() < n;) {
+        any = false;
+        for (int r = 0; r < 3 && out.size() < n; ++r)
+            if (at[r] < q[r].size()) {
+                out.push_back(q[r][at[r]++]);
+                any = true;
+            }
+    }
+    for (size_t j : p.of(-1))
+        if (out.size() < n) out.push_back(j);
+    return out;
+}
+
+// Region placement of a plan's tables (see above): pieces of a third of a
+// pass buffer's tables, up to six candidate ranges of one piece per region
+// (or, two regions seen, one piece of piece 0's region and two of the other,
+// from either end of their allocation order), each timed with k_place_walk at
+// the plan's own geometry; the two fastest become the table ranges.  *placed
+// stays false, with nothing allocated beyond ws, when it does not apply:
+// pieces too small, no virtual-memory support, fewer than two regions seen,
+// or too few pieces in them.
 static int place_regions(Ctx& c, const HbmDecision& d, bool* placed) {
     *placed = false;
     const size_t rest = d.h.off_tables, tabs = d.h.buffer - d.h.off_tables;
@@ -1666,208 +1841,83 @@ static int place_regions(Ctx& c, const HbmDecision& d, bool* placed) {
     const size_t room = c.budget > 2 * rest ? c.budget - 2 * rest : 0;
     const size_t maxp = std::min<size_t>(kRegionMaxPieces, room / P);
     if (P < kRegionMinPiece || maxp < 6) return CK_OK;
-    hipMemAllocationProp prop{};
-    prop.type = hipMemAllocationTypePinned;
-    prop.location.type = hipMemLocationTypeDevice;
-    prop.location.id = c.device;
-    hipMemAccessDesc acc{};
-    acc.location = prop.location;
-    acc.flags = hipMemAccessFlagsProtReadWrite;
-    char* pool = nullptr;
-    if (hipMemAddressReserve(reinterpret_cast<void**>(&pool), maxp * P, 0, nullptr, 0) != hipSuccess) {
-        (void)hipGetLastError();
-        return CK_OK;
-    }
-    std::vector<hipMemGenericAllocationHandle_t> h;
-    size_t mapped = 0, ready = 0;  // pieces mapped into the pool; of them, accessible
-    TableRange out[2];
-    int rc = CK_OK;
-    // Unmaps the pool and releases every piece not in out[].
-    auto drop_pool = [&]() {
-        (void)hipDeviceSynchronize();
-        for (size_t j = 0; j < mapped; ++j) (void)hipMemUnmap(pool + j * P, P);
-        (void)hipMemAddressFree(pool, maxp * P);
-        for (hipMemGenericAllocationHandle_t x : h) {
-            bool kept = false;
-            for (const TableRange& t : out)
-                for (hipMemGenericAllocationHandle_t y : t.pieces) kept |= x == y;
-            if (!kept) (void)hipMemRelease(x);
-        }
-    };
-    // Pair walks: half the lanes in piece a, half in piece b, 24 KB tables
-    // (768 MiB of each piece).
     auto* sink = reinterpret_cast<unsigned long long*>(c.ws.base);  // the first pass's counters, cleared per pass
-    auto pair_ms = [&](size_t a, size_t b, float* ms) {
-        return timed_twice(
-            c,
-            [&]() {
-                hipLaunchKernelGGL(k_place_pair, dim3(kRegionPairLanes / 256), dim3(256), 0, c.stream,
-                                   reinterpret_cast<uint64_t*>(pool + a * P), reinterpret_cast<uint64_t*>(pool + b * P),
-                                   kRegionPairLanes / 2, kRegionPairSlots, kRegionPairSteps, sink);
-            },
-            ms);
-    };
-    // t0: each piece paired with piece 0; t1: each piece of another region
-    // than piece 0's paired with ref1, the one most clearly apart from piece 0.
-    std::vector<float> t0, t1;
-    std::vector<int32_t> region;
-    size_t ref1 = 0;
-    bool two = false, full = false;
-    std::vector<size_t> of[3];
-    while (rc == CK_OK && !full) {
-        const size_t from = ready;
-        for (size_t k = 0; k < kRegionBatch && ready < maxp; ++k) {
-            hipMemGenericAllocationHandle_t x{};
-            if (hipMemCreate(&x, P, &prop, 0) != hipSuccess) {  // the device is full (or has no such API)
-                (void)hipGetLastError();
-                full = true;
-                break;
-            }
-            h.push_back(x);
-            hipError_t e = hipMemMap(pool + ready * P, P, 0, x, 0);
-            if (e == hipSuccess) {
-                ++mapped;
-                e = hipMemSetAccess(pool + ready * P, P, &acc, 1);
-            }
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                full = true;
-                break;
-            }
-            ++ready;
-        }
-        if (ready == maxp) full = true;
-        if (ready == from) break;
-        t0.resize(ready, 0.f);
-        t1.resize(ready, 0.f);
-        region.resize(ready, 0);
-        for (size_t j = std::max<size_t>(from, 1); j < ready && rc == CK_OK; ++j) rc = pair_ms(0, j, &t0[j]);
-        if (rc) break;
-        // Piece 0's region: near the slow mode; another: near the fast mode;
-        // in between: left out (-1).
-        std::vector<size_t> all;
-        for (size_t j = 1; j < ready; ++j) all.push_back(j);
-        float f0 = 0, s0 = 0;
-        two = split_modes(t0, all, &f0, &s0);
-        std::vector<size_t> others;
-        for (size_t j = 1; j < ready; ++j) {
-            region[j] = !two ? 0 : t0[j] >= s0 / kRegionNear ? 0 : t0[j] <= f0 * kRegionNear ? 1 : -1;
-            if (region[j] == 1) others.push_back(j);
-        }
-        if (two && !ref1 && !others.empty()) {
-            ref1 = others[0];
-            for (size_t j : others)
-                if (t0[j] < t0[ref1]) ref1 = j;
-        }
-        // A second split among the others, where it shows, is a third region.
-        std::vector<size_t> others1;
-        for (size_t j : others)
-            if (j != ref1) {
-                if (t1[j] == 0 && rc == CK_OK) rc = pair_ms(ref1, j, &t1[j]);
-                others1.push_back(j);
-            }
-        if (rc) break;
-        float f1 = 0, s1 = 0;
-        if (ref1 && split_modes(t1, others1, &f1, &s1))
-            for (size_t j : others1) region[j] = t1[j] >= s1 / kRegionNear ? 1 : t1[j] <= f1 * kRegionNear ? 2 : -1;
-        for (auto& v : of) v.clear();
-        for (size_t j = 0; j < ready; ++j)
-            if (region[j] >= 0) of[region[j]].push_back(j);
-        if (of[0].size() >= kRegionCands && of[1].size() >= kRegionCands && of[2].size() >= kRegionCands) break;
+    RegionPool rp;
+    std::string err;
+    if (int rc = region_pool(c.device, c.stream, c.ev_chain[0][0], c.ev_chain[0][1], sink, P, maxp, kRegionCands,
+                             &rp, &err)) {
+        drop_pool(rp, {});
+        return fail(rc, "table placement: " + err);
     }
-    // Candidate table ranges of three pieces, pairwise disjoint: one piece per
-    // region, or (two classes) one of the first region and two of the rest,
-    // from either end of their allocation order.
+    const std::vector<size_t> of[3] = {rp.of(0), rp.of(1), rp.of(2)};
     std::vector<std::array<size_t, 3>> cand;
-    if (two && of[1].size() >= 2 && of[2].size() >= 2) {
+    if (rp.two && of[1].size() >= 2 && of[2].size() >= 2) {
         const size_t k = std::min({size_t(kRegionCands), of[0].size(), of[1].size(), of[2].size()});
         for (size_t i = 0; i < k; ++i) cand.push_back({of[0][i], of[1][i], of[2][i]});
-    } else if (two) {
+    } else if (rp.two) {
         std::vector<size_t> rest1 = of[1];
         rest1.insert(rest1.end(), of[2].begin(), of[2].end());
         std::sort(rest1.begin(), rest1.end());
         const size_t k = std::min({size_t(kRegionCands), of[0].size(), rest1.size() / 2});
         for (size_t i = 0; i < k; ++i) cand.push_back({of[0][i], rest1[i], rest1[rest1.size() - 1 - i]});
     }
-    if (rc != CK_OK || cand.size() < 2) {
-        drop_pool();
-        return rc;
+    if (cand.size() < 2) {
+        drop_pool(rp, {});
+        return CK_OK;
     }
-    // Each candidate mapped into a test range and timed with the chains'
-    // access pattern at the plan's own geometry; the two fastest are kept.
     std::vector<float> cms(cand.size(), 0.f);
-    void* test = nullptr;
-    if (hipMemAddressReserve(&test, 3 * P, 0, nullptr, 0) != hipSuccess) {
-        drop_pool();
-        return fail(CK_EDEVICE, "hipMemAddressReserve for a test table range");
-    }
     const uint64_t pitch = d.h.table_bytes / 8;
+    int rc = CK_OK;
     for (size_t i = 0; i < cand.size() && rc == CK_OK; ++i) {
-        int maps = 0;
-        for (int r = 0; r < 3; ++r) {
-            if (hipMemMap(static_cast<char*>(test) + r * P, P, 0, h[cand[i][r]], 0) != hipSuccess) break;
-            ++maps;
-        }
-        if (maps < 3 || hipMemSetAccess(test, 3 * P, &acc, 1) != hipSuccess)
-            rc = fail(CK_EDEVICE, "mapping a test table range");
-        if (rc == CK_OK)
-            rc = timed_twice(
-                c,
+        void* test = nullptr;
+        rc = map_pieces(c.device, rp, {cand[i][0], cand[i][1], cand[i][2]}, &test, &err);
+        if (rc == CK_OK) {
+            const hipError_t e = timed_twice(
+                c.stream, c.ev_chain[0][0], c.ev_chain[0][1],
                 [&]() {
                     hipLaunchKernelGGL(k_place_walk, dim3((d.chunk + 255) / 256), dim3(256), 0, c.stream,
                                        static_cast<uint64_t*>(test), pitch, static_cast<uint32_t>(pitch), d.chunk,
                                        kPlaceWalkSteps, sink);
                 },
                 &cms[i]);
-        (void)hipDeviceSynchronize();
-        for (int r = 0; r < maps; ++r) (void)hipMemUnmap(static_cast<char*>(test) + r * P, P);
+            if (e != hipSuccess) {
+                rc = CK_EDEVICE;
+                err = std::string("walk over a candidate range: ") + hipGetErrorString(e);
+            }
+            (void)hipDeviceSynchronize();
+            unmap_pieces(test, P, 3);
+        }
     }
-    (void)hipMemAddressFree(test, 3 * P);
     std::vector<int> order(cand.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cms[x] < cms[y]; });
-    int32_t used[6];
-    for (int b = 0; b < 2; ++b)
-        for (int r = 0; r < 3; ++r) used[3 * b + r] = static_cast<int32_t>(cand[order[b]][r]);
-    if (rc != CK_OK) {
-        drop_pool();
-        return rc;
-    }
+    TableRange out[2];
     for (int b = 0; b < 2 && rc == CK_OK; ++b) {
-        TableRange& t = out[b];
+        const std::array<size_t, 3>& pick = cand[order[b]];
         void* va = nullptr;
-        if (hipMemAddressReserve(&va, 3 * P, 0, nullptr, 0) != hipSuccess) {
-            rc = fail(CK_EDEVICE, "hipMemAddressReserve for the table range");
-            break;
-        }
-        t.base = va;
-        t.bytes = 3 * P;
-        t.piece = P;
-        for (int i = 0; i < 3 && rc == CK_OK; ++i) {
-            if (hipMemMap(static_cast<char*>(va) + i * P, P, 0, h[used[3 * b + i]], 0) != hipSuccess)
-                rc = fail(CK_EDEVICE, "hipMemMap of a table piece");
-            else
-                t.pieces.push_back(h[used[3 * b + i]]);
-        }
-        if (rc == CK_OK && hipMemSetAccess(va, 3 * P, &acc, 1) != hipSuccess) rc = fail(CK_EDEVICE, "hipMemSetAccess");
+        rc = map_pieces(c.device, rp, {pick[0], pick[1], pick[2]}, &va, &err);
+        if (rc) break;
+        out[b].base = va;
+        out[b].bytes = 3 * P;
+        out[b].piece = P;
+        for (size_t j : pick) out[b].pieces.push_back(rp.h[j]);
     }
     if (rc != CK_OK) {
         (void)hipDeviceSynchronize();
-        for (TableRange& t : out) {
-            for (size_t i = 0; i < t.pieces.size(); ++i) (void)hipMemUnmap(static_cast<char*>(t.base) + i * P, P);
-            if (t.base) (void)hipMemAddressFree(t.base, t.bytes);
-            t = TableRange{};
-        }
-        drop_pool();
-        return rc;
+        for (TableRange& t : out) unmap_pieces(t.base, P, t.pieces.size());
+        drop_pool(rp, {});
+        return fail(rc, "table placement: " + err);
     }
-    drop_pool();
+    std::vector<hipMemGenericAllocationHandle_t> keep = out[0].pieces;
+    keep.insert(keep.end(), out[1].pieces.begin(), out[1].pieces.end());
+    c.place_region = rp.region;
+    drop_pool(rp, keep);
     c.tabs[0] = std::move(out[0]);
     c.tabs[1] = std::move(out[1]);
     c.placed_for = tabs;
     c.place_ms = cms;
-    c.place_region = region;
-    std::copy(used, used + 6, c.place_used);
+    for (int b = 0; b < 2; ++b)
+        for (int r = 0; r < 3; ++r) c.place_used[3 * b + r] = static_cast<int32_t>(cand[order[b]][r]);
     c.place_kept[0] = order[0];
     c.place_kept[1] = order[1];
     forget_tables(c);
@@ -1919,18 +1969,18 @@ static int place_buffers(Ctx& c, const HbmDecision& d) {
     auto* sink = reinterpret_cast<unsigned long long*>(c.ws.base);
     std::vector<float> ms(cand.size(), 0.f);
     for (size_t i = 0; i < cand.size(); ++i) {
-        const int rc = timed_twice(
-            c,
+        const hipError_t e = timed_twice(
+            c.stream, c.ev_chain[0][0], c.ev_chain[0][1],
             [&]() {
                 hipLaunchKernelGGL(k_place_walk, dim3((lanes + 255) / 256), dim3(256), 0, c.stream,
                                    static_cast<uint64_t*>(cand[i]), pitch, static_cast<uint32_t>(pitch), lanes,
                                    kPlaceWalkSteps, sink);
             },
             &ms[i]);
-        if (rc) {
+        if (e != hipSuccess) {
             (void)hipDeviceSynchronize();
             for (void* x : cand) (void)hipFree(x);
-            return rc;
+            return fail(CK_EDEVICE, std::string("walk over a candidate table allocation: ") + hipGetErrorString(e));
         }
     }
     std::vector<int> order(cand.size());

@@ -1,0 +1,69 @@
+// ck_regions.hpp -- the device's memory regions, found by pair walks.
+//
+// MI355X device memory falls into three regions of about 96 GB, and random
+// read + write traffic confined to one region runs slower than traffic spread
+// over several (a dependent random read-write walk over 65,536 tables: 20.5 G
+// steps/s in one region, 27.3 G spread evenly over three).  Ordinary
+// allocations fill the regions in turn, so a buffer of a few GB lies in one.
+// These helpers allocate memory in pieces through the virtual-memory API,
+// sort the pieces into regions and map chosen pieces into one range: the
+// table placement of ck_ctx_set_place_search and the engine's spread arena
+// are built on them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ck {
+
+// Pieces of one size mapped one after another into `pool` (piece j at
+// pool + j * piece), each sorted into a region: 0 is piece 0's, 1 and 2 are
+// others, -1 is unclear (piece 0's neighbours, pieces straddling a boundary).
+struct RegionPool {
+    size_t piece = 0;
+    char* pool = nullptr;
+    size_t reserved = 0;  // bytes of address space reserved at pool
+    size_t mapped = 0;    // pieces mapped into the pool (the first `ready` of them accessible)
+    size_t ready = 0;
+    std::vector<hipMemGenericAllocationHandle_t> h;
+    std::vector<int32_t> region;
+    bool two = false;     // two regions or more seen
+    std::vector<size_t> of(int r) const {
+        std::vector<size_t> v;
+        for (size_t j = 0; j < ready; ++j)
+            if (region[j] == r) v.push_back(j);
+        return v;
+    }
+};
+
+// Allocates pieces of `piece` bytes (a multiple of 2 MiB, 1 GiB or more: the
+// pair walk covers 768 MiB of each) in batches, at most `max_pieces`, until
+// every region holds `want` of them or the device is full, and sorts them by
+// pair walks launched on `stream` and timed with e0 / e1 (`sink`: 8 writable
+// device bytes).  Returns CK_OK with out->ready == 0 where the virtual-memory
+// API is missing; *err describes a failure.
+int region_pool(int device, hipStream_t stream, hipEvent_t e0, hipEvent_t e1, unsigned long long* sink,
+                size_t piece, size_t max_pieces, size_t want, RegionPool* out, std::string* err);
+
+// Maps pieces `pick` of the pool one after another into a newly reserved
+// range (*va, pick.size() * piece bytes), readable and writable on `device`.
+int map_pieces(int device, const RegionPool& p, const std::vector<size_t>& pick, void** va, std::string* err);
+
+// Unmaps the first n pieces of a range made by map_pieces and frees its
+// address space (the pieces themselves stay allocated).
+void unmap_pieces(void* va, size_t piece, size_t n);
+
+// Unmaps the pool, frees its address space and releases every piece not in
+// `keep`.  Synchronizes the device first.
+void drop_pool(RegionPool& p, const std::vector<hipMemGenericAllocationHandle_t>& keep);
+
+// Interleaved order for a range of n pieces: regions in turn (0, 1, 2, 0, ...)
+// while each has pieces left, then whatever remains (clear pieces before
+// unclear ones).  Fewer than n when the pool holds fewer.
+std::vector<size_t> interleave(const RegionPool& p, size_t n);
+
+}  // namespace ck

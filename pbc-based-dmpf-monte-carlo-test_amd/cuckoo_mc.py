@@ -55,7 +55,7 @@ class JobResult(C.Structure):
 
 class EngineConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_uint64), ("buffers", C.c_uint32), ("max_slots", C.c_uint32),
-                ("max_jobs", C.c_uint32), ("policy", C.c_uint32), ("window", C.c_uint32), ("pad", C.c_uint32)]
+                ("max_jobs", C.c_uint32), ("policy", C.c_uint32), ("window", C.c_uint32), ("spread", C.c_uint32)]
 
 
 class EngineStats(C.Structure):
@@ -139,6 +139,7 @@ def _load():
         "ck_engine_failures": (C.c_int, [vp, U64, P(U64), U64, P(U64)]),
         "ck_engine_release": (C.c_int, [vp, U64]),
         "ck_engine_stats_get": (C.c_int, [vp, P(EngineStats)]),
+        "ck_engine_arena": (C.c_int, [vp, P(U32)]),
         "ck_trial_inputs_of": (C.c_int, [P(Params), P(U32), U32, P(TrialInputs)]),
         "ck_round_cache": (C.c_int, [P(Params), P(U32), P(C.c_uint16)]),
         "ck_ctx_set_placement": (C.c_int, [vp, P(Placement)]),
@@ -162,7 +163,7 @@ EXPORTS = ["ck_params_derive", "ck_batch_key", "ck_host_aes", "ck_ctx_create", "
            "ck_engine_destroy", "ck_engine_submit", "ck_engine_poll", "ck_engine_wait", "ck_engine_failures",
            "ck_engine_release", "ck_engine_stats_get", "ck_trial_inputs_of", "ck_round_cache",
            "ck_ctx_set_placement", "ck_ctx_plan", "ck_plan_describe", "ck_launch_static",
-           "ck_ctx_set_place_search", "ck_ctx_place_report", "ck_ctx_place_regions"]
+           "ck_ctx_set_place_search", "ck_ctx_place_report", "ck_ctx_place_regions", "ck_engine_arena"]
 
 
 def _check(rc):
@@ -362,8 +363,8 @@ class Engine:
     are the same trial-for-trial as Context.launch over the same ranges."""
 
     def __init__(self, device: int = 0, arena_bytes: int = 0, buffers: int = 0, max_slots: int = 0,
-                 max_jobs: int = 0, policy: int = SCHED_SRPT, window: int = 0):
-        cfg = EngineConfig(arena_bytes, buffers, max_slots, max_jobs, policy, window, 0)
+                 max_jobs: int = 0, policy: int = SCHED_SRPT, window: int = 0, spread: bool = False):
+        cfg = EngineConfig(arena_bytes, buffers, max_slots, max_jobs, policy, window, int(spread))
         h = C.c_void_p()
         _check(LIB.ck_engine_create(device, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -415,6 +416,13 @@ class Engine:
 
     def release(self, job_id: int):
         _check(LIB.ck_engine_release(self.h, job_id))
+
+    def arena_regions(self):
+        """Pieces of the arena per memory region (0, 1, 2, unclear); all zero
+        when the arena is one allocation."""
+        counts = (C.c_uint32 * 4)()
+        _check(LIB.ck_engine_arena(self.h, counts))
+        return tuple(counts)
 
     def stats(self) -> EngineStats:
         s = EngineStats()

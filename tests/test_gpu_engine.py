@@ -187,3 +187,23 @@ def test_engine_bucket_map_group():
         r = eng.wait(jid)
         assert list(map(int, eng.failures(jid))) == ofails
         assert r.probes == oprobes
+
+
+def test_engine_spread_arena_matches_oracle(expected):
+    """A spread arena (1 GiB pieces taken from the memory regions in turn):
+    the pieces cover two regions or more, and every job's results are the
+    oracle's."""
+    with ck.Engine(0, arena_bytes=24 * GiB, buffers=3, max_slots=2048, spread=True) as eng:
+        counts = eng.arena_regions()
+        got = run_jobs(eng, JOBS)
+    assert sum(counts) == 24 and sum(1 for c in counts[:3] if c) >= 2, counts
+    assert max(counts[:3]) - min(c for c in counts[:3] if c) <= 1, counts  # the regions in turn
+    for (fails, r), (ofails, oprobes), j in zip(got, expected, JOBS):
+        assert fails == ofails and r.probes == oprobes, j
+
+
+def test_engine_without_spread_reports_no_pieces():
+    with ck.Engine(0, arena_bytes=GiB, max_slots=1024) as eng:
+        assert eng.arena_regions() == (0, 0, 0, 0)
+    with pytest.raises(ck.CkError):
+        ck.Engine(0, arena_bytes=GiB, spread=2)
