@@ -1625,36 +1625,6 @@ static hipError_t timed_twice(hipStream_t s, hipEvent_t e0, hipEvent_t e1, F&& l
     return hipSuccess;
 }
 
-// Two modes among the times t[idx]: 2-means on their logarithms (so a few
-// pieces in between, such as the reference's neighbours or pieces straddling
-// a boundary, do not set the modes), each mode the median of its cluster.
-// False unless both clusters hold two or more and the slow mode is at least
-// kRegionModes times the fast one.
-static bool split_modes(const std::vector<float>& t, const std::vector<size_t>& idx, float* fast, float* slow) {
-    std::vector<float> v;
-    for (size_t j : idx)
-        if (t[j] > 0) v.push_back(std::log(t[j]));
-    if (v.size() < 4) return false;
-    std::sort(v.begin(), v.end());
-    float lo = v.front(), hi = v.back();
-    size_t cut = 0;  // v[0, cut) fast, v[cut, n) slow
-    for (int it = 0; it < 32; ++it) {
-        const float mid = 0.5f * (lo + hi);
-        cut = static_cast<size_t>(std::upper_bound(v.begin(), v.end(), mid) - v.begin());
-        if (cut == 0 || cut == v.size()) return false;
-        float sl = 0, sh = 0;
-        for (size_t i = 0; i < v.size(); ++i) (i < cut ? sl : sh) += v[i];
-        const float nlo = sl / cut, nhi = sh / (v.size() - cut);
-        if (nlo == lo && nhi == hi) break;
-        lo = nlo;
-        hi = nhi;
-    }
-    if (cut < 2 || v.size() - cut < 2) return false;
-    *fast = std::exp(v[cut / 2]);
-    *slow = std::exp(v[cut + (v.size() - cut) / 2]);
-    return *slow >= *fast * kRegionModes;
-}
-
 static void region_access(int device, hipMemAllocationProp* prop, hipMemAccessDesc* acc) {
     *prop = hipMemAllocationProp{};
     prop->type = hipMemAllocationTypePinned;
@@ -1741,7 +1711,7 @@ int region_pool(int device, hipStream_t stream, hipEvent_t e0, hipEvent_t e1, un
         std::vector<size_t> all;
         for (size_t j = 1; j < n; ++j) all.push_back(j);
         float f0 = 0, s0 = 0;
-        R.two = split_modes(t0, all, &f0, &s0);
+        R.two = split_modes(t0, all, kRegionModes, &f0, &s0);
         std::vector<size_t> others;
         for (size_t j = 1; j < n; ++j) {
             R.region[j] = !R.two ? 0 : t0[j] >= s0 / kRegionNear ? 0 : t0[j] <= f0 * kRegionNear ? 1 : -1;
@@ -1763,7 +1733,7 @@ int region_pool(int device, hipStream_t stream, hipEvent_t e0, hipEvent_t e1, un
                 others1.push_back(j);
             }
         float f1 = 0, s1 = 0;
-        if (ref1 && split_modes(t1, others1, &f1, &s1))
+        if (ref1 && split_modes(t1, others1, kRegionModes, &f1, &s1))
             for (size_t j : others1) R.region[j] = t1[j] >= s1 / kRegionNear ? 1 : t1[j] <= f1 * kRegionNear ? 2 : -1;
         if (R.of(0).size() >= want && R.of(1).size() >= want && R.of(2).size() >= want) break;
     }
@@ -1807,23 +1777,6 @@ void drop_pool(RegionPool& p, const std::vector<hipMemGenericAllocationHandle_t>
     for (hipMemGenericAllocationHandle_t x : p.h)
         if (std::find(keep.begin(), keep.end(), x) == keep.end()) (void)hipMemRelease(x);
     p = RegionPool{};
-}
-
-std::vector<size_t> interleave(const RegionPool& p, size_t n) {
-    const std::vector<size_t> q[3] = {p.of(0), p.of(1), p.of(2)};
-    size_t at[3] = {0, 0, 0};
-    std::vector<size_t> out;
-    for (bool any = true; any && out.size() < n;) {
-        any = false;
-        for (int r = 0; r < 3 && out.size() < n; ++r)
-            if (at[r] < q[r].size()) {
-                out.push_back(q[r][at[r]++]);
-                any = true;
-            }
-    }
-    for (size_t j : p.of(-1))
-        if (out.size() < n) out.push_back(j);
-    return out;
 }
 
 // Region placement of a plan's tables (see above): pieces of a third of a

@@ -44,6 +44,8 @@
 #include <string>
 #include <vector>
 
+#include "ck_regions.hpp"
+
 namespace {
 
 constexpr uint32_t kGolden32 = 0x9E3779B9u;
@@ -128,31 +130,6 @@ __global__ __launch_bounds__(256) void k_pair(uint64_t* __restrict__ a, uint64_t
 
 }  // namespace
 
-// Two modes among the times v: 2-means on their logarithms, each mode the
-// median of its cluster.  False unless both clusters hold two or more and the
-// slow mode is at least 1.12 times the fast one.
-static bool split_modes(std::vector<float> v, float* fast, float* slow) {
-    for (float& x : v) x = std::log(x);
-    if (v.size() < 4) return false;
-    std::sort(v.begin(), v.end());
-    float lo = v.front(), hi = v.back();
-    size_t cut = 0;
-    for (int it = 0; it < 32; ++it) {
-        cut = static_cast<size_t>(std::upper_bound(v.begin(), v.end(), 0.5f * (lo + hi)) - v.begin());
-        if (cut == 0 || cut == v.size()) return false;
-        float sl = 0, sh = 0;
-        for (size_t i = 0; i < v.size(); ++i) (i < cut ? sl : sh) += v[i];
-        const float nlo = sl / cut, nhi = sh / (v.size() - cut);
-        if (nlo == lo && nhi == hi) break;
-        lo = nlo;
-        hi = nhi;
-    }
-    if (cut < 2 || v.size() - cut < 2) return false;
-    *fast = std::exp(v[cut / 2]);
-    *slow = std::exp(v[cut + (v.size() - cut) / 2]);
-    return *slow >= *fast * 1.12f;
-}
-
 // A range of `bytes` made of three virtual-memory pieces from three memory
 // regions (two where only two show), found as the library's table placement
 // finds them: pieces allocated in batches, each paired with piece 0 and, if
@@ -216,7 +193,9 @@ static int spread_range(size_t bytes, unsigned long long* sink, hipEvent_t a, hi
         for (size_t j = std::max<size_t>(from, 1); j < h.size(); ++j)
             if (pair(0, j, &t0[j])) return 1;
         float f0 = 0, s0 = 0, f1 = 0, s1 = 0;
-        if (!split_modes(std::vector<float>(t0.begin() + 1, t0.end()), &f0, &s0)) continue;
+        std::vector<size_t> all;
+        for (size_t j = 1; j < h.size(); ++j) all.push_back(j);
+        if (!ck::split_modes(t0, all, 1.12f, &f0, &s0)) continue;
         std::vector<size_t> others;
         for (size_t j = 1; j < h.size(); ++j) {
             region[j] = t0[j] >= s0 / 1.06f ? 0 : t0[j] <= f0 * 1.06f ? 1 : -1;
@@ -227,13 +206,13 @@ static int spread_range(size_t bytes, unsigned long long* sink, hipEvent_t a, hi
             for (size_t j : others)
                 if (t0[j] < t0[ref1]) ref1 = j;
         }
-        std::vector<float> v1;
+        std::vector<size_t> others1;
         for (size_t j : others)
             if (j != ref1) {
                 if (t1[j] == 0 && pair(ref1, j, &t1[j])) return 1;
-                v1.push_back(t1[j]);
+                others1.push_back(j);
             }
-        if (split_modes(v1, &f1, &s1))
+        if (ck::split_modes(t1, others1, 1.12f, &f1, &s1))
             for (size_t j : others)
                 if (j != ref1) region[j] = t1[j] >= s1 / 1.06f ? 1 : t1[j] <= f1 * 1.06f ? 2 : -1;
         pick[1] = pick[2] = -1;

@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -62,8 +64,54 @@ void unmap_pieces(void* va, size_t piece, size_t n);
 void drop_pool(RegionPool& p, const std::vector<hipMemGenericAllocationHandle_t>& keep);
 
 // Interleaved order for a range of n pieces: regions in turn (0, 1, 2, 0, ...)
-// while each has pieces left, then whatever remains (clear pieces before
-// unclear ones).  Fewer than n when the pool holds fewer.
-std::vector<size_t> interleave(const RegionPool& p, size_t n);
+// while each has pieces left, then the unclear ones.  Fewer than n when the
+// pool holds fewer.
+inline std::vector<size_t> interleave(const RegionPool& p, size_t n) {
+    const std::vector<size_t> q[3] = {p.of(0), p.of(1), p.of(2)};
+    size_t at[3] = {0, 0, 0};
+    std::vector<size_t> out;
+    for (bool any = true; any && out.size() < n;) {
+        any = false;
+        for (int r = 0; r < 3 && out.size() < n; ++r)
+            if (at[r] < q[r].size()) {
+                out.push_back(q[r][at[r]++]);
+                any = true;
+            }
+    }
+    for (size_t j : p.of(-1))
+        if (out.size() < n) out.push_back(j);
+    return out;
+}
+
+// Two modes among the times t[idx]: 2-means on their logarithms (so a few
+// pieces in between, such as the reference's neighbours or pieces straddling
+// a boundary, do not set the modes), each mode the median of its cluster.
+// False unless both clusters hold two or more and the slow mode is at least
+// `ratio` times the fast one.
+inline bool split_modes(const std::vector<float>& t, const std::vector<size_t>& idx, float ratio, float* fast,
+                        float* slow) {
+    std::vector<float> v;
+    for (size_t j : idx)
+        if (t[j] > 0) v.push_back(std::log(t[j]));
+    if (v.size() < 4) return false;
+    std::sort(v.begin(), v.end());
+    float lo = v.front(), hi = v.back();
+    size_t cut = 0;  // v[0, cut) fast, v[cut, n) slow
+    for (int it = 0; it < 32; ++it) {
+        const float mid = 0.5f * (lo + hi);
+        cut = static_cast<size_t>(std::upper_bound(v.begin(), v.end(), mid) - v.begin());
+        if (cut == 0 || cut == v.size()) return false;
+        float sl = 0, sh = 0;
+        for (size_t i = 0; i < v.size(); ++i) (i < cut ? sl : sh) += v[i];
+        const float nlo = sl / cut, nhi = sh / (v.size() - cut);
+        if (nlo == lo && nhi == hi) break;
+        lo = nlo;
+        hi = nhi;
+    }
+    if (cut < 2 || v.size() - cut < 2) return false;
+    *fast = std::exp(v[cut / 2]);
+    *slow = std::exp(v[cut + (v.size() - cut) / 2]);
+    return *slow >= *fast * ratio;
+}
 
 }  // namespace ck
