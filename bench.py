@@ -711,8 +711,8 @@ def main():
     if args.chunk:
         ctx.set_chunk(args.chunk)
     ctx.set_layout(layout)
-    # Pass buffers: the two fastest of up to 16 candidate allocations (the
-    # chain rate belongs to the memory a buffer lands in).
+    # Tables spread over the device's memory regions (the chain rate belongs
+    # to the memory its tables land in).
     ctx.set_place_search(args.place_search)
     # With every placement knob left to the library, the context's placement
     # (tier, table layout, trials per pass) is the autotuner's pick on a sample

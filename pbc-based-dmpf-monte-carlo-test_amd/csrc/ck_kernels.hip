@@ -333,7 +333,7 @@ __global__ __launch_bounds__(64) void k_chain_lane(Shape S, uint32_t first, uint
     add_wave_total(pr, reinterpret_cast<unsigned long long*>(counters + CNT_PROBES));
 }
 
-// Placement search: the insertion chains' access pattern without their logic
+// Table placement: the insertion chains' access pattern without their logic
 // -- each of `lanes` lanes walks its own table of `slots` 8-byte words (pitch
 // words apart), reading a word and writing it back changed, the next index
 // taken from the value read.  The contents are whatever the memory holds
@@ -2568,7 +2568,7 @@ int ck_ctx_tune(ck_ctx* ctx, const ck_params* p, const uint32_t rk[44], uint32_t
         c.chunk_cap = k.chunk;
         uint32_t nf = 0;
         // Warm-up: a launch planned like the timed one (the same trials per
-        // pass), so that workspace growth and the placement search happen here.
+        // pass), so that workspace growth and the table placement happen here.
         rc = launch_common(ctx, p, rk, 0, std::min(sample, kHbmChunk), got.data(), &nf, nullptr, nullptr, nullptr,
                            nullptr);
         const auto t0 = std::chrono::steady_clock::now();

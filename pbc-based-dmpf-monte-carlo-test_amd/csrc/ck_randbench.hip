@@ -338,7 +338,7 @@ int main(int argc, char** argv) {
     CHECK(hipFree(buf));
     if (search) {
         // Candidate buffers for 65,536 lanes, all held at once (as the library's
-        // pass-buffer search holds its candidates), as many as fit 90 % of the
+        // fallback search holds its candidates), as many as fit 90 % of the
         // free memory.
         const uint32_t lanes = 65536;
         const uint64_t bytes = pitch * lanes * 8;

@@ -48,7 +48,7 @@ struct RunSpec {
     uint32_t block_threads = 256, warps_per_block = 0;
     std::string prp_cache = "auto", table_layout = "auto", shared_variant = "auto";
     std::string tier = "auto";
-    uint32_t place_search = 32;        // candidate table regions per placement search (0: none)
+    uint32_t place_search = 32;        // table placement on, candidates of its fallback search (0: off)
     bool global_table = false, status_only = false, tune = false, graph = false;
 };
 

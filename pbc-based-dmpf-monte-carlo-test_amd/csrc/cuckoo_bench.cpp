@@ -100,7 +100,8 @@ void worker_main(Shared& sh, int index, int device) {
     check(ck_ctx_set_placement(ctx.get(), &want), "ck_ctx_set_placement");
     check(ck_ctx_set_graph(ctx.get(), s.graph ? 1 : 0), "ck_ctx_set_graph");
     // A worker runs many passes of one shape: the tables of its pass buffers
-    // are the two fastest of --place-search candidates (where the plan qualifies).
+    // are spread over the memory regions (or, where pieces would be small, the
+    // two fastest of --place-search candidates) where the plan qualifies.
     check(ck_ctx_set_place_search(ctx.get(), s.place_search), "ck_ctx_set_place_search");
     const uint32_t micro = static_cast<uint32_t>(std::min<uint64_t>(s.micro, uint64_t(1) << s.bt));
     std::string tuned;
