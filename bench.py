@@ -74,13 +74,13 @@ N20_PROFILE = dict(kernel="k_chain_res<3, bgh, u64 candidate stream, 44-bit entr
                    requests=0.897 + 0.885, l2_hit_rate=0.686, profile="profiles/k_chain_res_hbm_n2_20_counters.md")
 # DRAM requests per step of ck_randbench (profiles/randbench_192k_traffic.md).
 RB_READ_WRITE_REQ_PER_STEP = 0.998 + 1.0005
-# Engines of the sweep and serving legs take their arenas spread over the
-# device's memory regions (--no-spread: one allocation).
-SPREAD_ARENA = True
 RB_READ_ONLY_REQ_PER_STEP = 0.998
 RB_READ_ONLY_W4_REQ_PER_STEP = 0.9636  # four walks share a lane's table: a few more L2 hits
 RB_READ_WRITE_W4_REQ_PER_STEP = 0.9989 + 0.9998
 RB_READ_WRITE_7OF8_REQ_PER_STEP = 0.9977 + 0.8062  # seven steps in eight write back
+# Engines of the sweep and serving legs take their arenas spread over the
+# device's memory regions (--no-spread: one allocation).
+SPREAD_ARENA = True
 
 
 def int_ops_per_trial(p, probes_per_trial: float) -> float:
