@@ -41,6 +41,19 @@ __constant__ uint32_t c_te0[256];
 //   [0] failures  [2..3] probe total (u64)  [4] LDS-tier work cursor
 constexpr uint32_t CNT_FAIL = 0, CNT_PROBES = 2, CNT_CURSOR = 4, CNT_WORDS = 16;
 
+// Where a pass's per-slot tables lie.  n = 1: one range, slot s at base[0] +
+// s * bytes.  n = 3: three ranges, one per memory region, slot s in range
+// s mod 3 at (s / 3) * bytes, so that neighbouring slots -- the lanes of a
+// wave, and whichever slots are still running at a pass's end -- spread their
+// random accesses over the three regions.
+struct TabRef {
+    char* base[3];
+    uint32_t n;
+    __host__ __device__ char* at(size_t s, size_t bytes) const {
+        return n == 1 ? base[0] + s * bytes : base[s % 3] + (s / 3) * bytes;
+    }
+};
+
 // ------------------------------------------------------------------ kernels
 
 // Stages T0 (2^shift copies) into LDS; returns the lane's view.
@@ -62,7 +75,7 @@ __device__ __forceinline__ LdsTab stage_tab(uint32_t* lds, uint32_t shift) {
 template <int K, bool CACHED, class M>
 __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys bk, uint32_t first, uint32_t count,
                                                uint32_t tab_shift, M* __restrict__ map, size_t map_stride,
-                                               char* __restrict__ tables, size_t table_bytes,
+                                               TabRef tables, size_t table_bytes,
                                                uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
                                                uint32_t* __restrict__ key_out, uint32_t clear, uint32_t item0,
                                                uint32_t item1) {
@@ -98,7 +111,7 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys
         }
     }
 
-    if (clear) clear_table(tables + slot * table_bytes, table_bytes, lane);
+    if (clear) clear_table(tables.at(slot, table_bytes), table_bytes, lane);
     if (lane == 0 && item0 == 0) {
         walks[slot] = ts.walk;
         rnds[slot] = ts.rnd;
@@ -112,7 +125,7 @@ __global__ __launch_bounds__(1024) CK_BUILD_REGS void k_build(Shape S, RoundKeys
 template <int K, bool BGH, class M, class E, bool XCHG>
 __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t count,
                                                const M* __restrict__ map, size_t map_stride,
-                                               char* __restrict__ tables, size_t table_bytes,
+                                               TabRef tables, size_t table_bytes,
                                                const uint64_t* __restrict__ walks, const uint64_t* __restrict__ rnds,
                                                uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
                                                uint32_t* __restrict__ probes_out, uint64_t tag) {
@@ -121,7 +134,7 @@ __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t
     uint32_t pr = 0;
     if (slot < count) {
         const M* m = map + slot * map_stride;
-        E* t = reinterpret_cast<E*>(tables + slot * table_bytes);
+        E* t = reinterpret_cast<E*>(tables.at(slot, table_bytes));
         if constexpr (sizeof(M) == 8) {
             const DescCodec D{S.dbits, K * S.dbits, (1ull << S.dbits) - 1, (1ull << (K * S.dbits)) - 1,
                               K * S.dbits + 2, XCHG ? 0ull : tag};
@@ -157,12 +170,12 @@ __global__ __launch_bounds__(256) void k_chain(Shape S, uint32_t first, uint32_t
 // Debug: per-slot digest of the final table (slot_digest over occupied buckets).
 template <int K, class M, class E>
 __global__ __launch_bounds__(256) void k_digest(Shape S, uint32_t count, const M* __restrict__ map, size_t map_stride,
-                                                const char* __restrict__ tables, size_t table_bytes,
+                                                TabRef tables, size_t table_bytes,
                                                 unsigned long long* __restrict__ out) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= count) return;
     const M* m = map + slot * map_stride;
-    const E* t = reinterpret_cast<const E*>(tables + slot * table_bytes);
+    const E* t = reinterpret_cast<const E*>(tables.at(slot, table_bytes));
     uint64_t sum = 0;
     for (uint32_t b = 0; b < S.buckets; ++b) {
         const E e = t[b];
@@ -200,7 +213,7 @@ __global__ __launch_bounds__(256) void k_digest(Shape S, uint32_t count, const M
 template <int K, bool BGH, class M, class E>
 __global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint32_t count,
                                                    const M* __restrict__ map, size_t map_stride,
-                                                   char* __restrict__ tables, size_t table_bytes,
+                                                   TabRef tables, size_t table_bytes,
                                                    uint64_t* __restrict__ walks, uint64_t* __restrict__ rnds,
                                                    uint32_t* __restrict__ counters, uint32_t* __restrict__ fail_offsets,
                                                    uint32_t* __restrict__ probes_out, uint32_t item0, uint32_t item1,
@@ -217,7 +230,7 @@ __global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint
         if (!(flags >> 31)) {
             const ResCodec<K, typename ResWord<E>::type> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
             const CandStream<K, M> st{map + slot * map_stride, S.dbits, (1ull << S.dbits) - 1};
-            const auto t = res_table<E>(tables + slot * table_bytes);
+            const auto t = res_table<E>(tables.at(slot, table_bytes));
             const ItemSpan span{item0, item1};
             uint32_t stash_left = flags;
             if constexpr (BGH) {
@@ -240,13 +253,13 @@ __global__ __launch_bounds__(256) void k_chain_res(Shape S, uint32_t first, uint
 
 // Debug: per-slot digest of a residual-layout table.
 template <int K, class E>
-__global__ __launch_bounds__(256) void k_digest_res(Shape S, uint32_t count, const char* __restrict__ tables,
+__global__ __launch_bounds__(256) void k_digest_res(Shape S, uint32_t count, TabRef tables,
                                                     size_t table_bytes, unsigned long long* __restrict__ out) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= count) return;
     using W = typename ResWord<E>::type;
     const ResCodec<K, W> R{S.dbits, (K - 1) * S.dbits, (1ull << S.dbits) - 1};
-    const auto t = res_table<E>(const_cast<char*>(tables) + slot * table_bytes);
+    const auto t = res_table<E>(tables.at(slot, table_bytes));
     uint64_t sum = 0;
     for (uint32_t b = 0; b < S.buckets; ++b) {
         const W e = t[b];
@@ -334,16 +347,16 @@ __global__ __launch_bounds__(64) void k_chain_lane(Shape S, uint32_t first, uint
 }
 
 // Table placement: the insertion chains' access pattern without their logic
-// -- each of `lanes` lanes walks its own table of `slots` 8-byte words (pitch
-// words apart), reading a word and writing it back changed, the next index
-// taken from the value read.  The contents are whatever the memory holds
-// (every index is below `slots`); `sink` only keeps the loads live.
-__global__ __launch_bounds__(256) void k_place_walk(uint64_t* __restrict__ base, uint64_t pitch, uint32_t slots,
-                                                    uint32_t lanes, uint32_t steps,
-                                                    unsigned long long* __restrict__ sink) {
+// -- each of `lanes` lanes walks its own table of `slots` 8-byte words (lane
+// l's at tables.at(l, pitch * 8)), reading a word and writing it back changed,
+// the next index taken from the value read.  The contents are whatever the
+// memory holds (every index is below `slots`); `sink` only keeps the loads
+// live.
+__global__ __launch_bounds__(256) void k_place_walk(TabRef tables, uint64_t pitch, uint32_t slots, uint32_t lanes,
+                                                    uint32_t steps, unsigned long long* __restrict__ sink) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= lanes) return;
-    uint64_t* t = base + uint64_t(lane) * pitch;
+    uint64_t* t = reinterpret_cast<uint64_t*>(tables.at(lane, pitch * 8));
     uint32_t ctr = lane * 0x9E3779B9u;
     uint32_t i = __umulhi(lane * 0x85EBCA6Bu + 0x27D4EB2Fu, slots);
     uint64_t acc = 0;
@@ -1061,7 +1074,7 @@ struct PassArgs {
     uint32_t first, count;
     char* map;
     size_t map_stride;
-    char* tables;
+    TabRef tables;
     size_t table_bytes;
     uint64_t* walks;
     uint64_t* rnds;
@@ -1588,8 +1601,8 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
 // (ck_ctx_set_place_search turns it on; the CLI and the headline bench do).
 // Results do not depend on it: every pass computes the same trials anywhere.
 // Placed plans: passes of at least kPlaceMinChains chains (a walk over fewer
-// lanes does not load the random-access path) in buffers of at least 1 GiB.
-static constexpr size_t kPlaceMinBuffer = size_t(1) << 30;
+// lanes does not load the random-access path) in buffers of at least 256 MiB.
+static constexpr size_t kPlaceMinBuffer = size_t(256) << 20;
 static constexpr uint32_t kPlaceWalkSteps = 4096;
 static constexpr uint32_t kPlaceMinChains = 1u << 14;
 static constexpr size_t kRegionMinPiece = size_t(1) << 30;
@@ -1601,8 +1614,9 @@ static constexpr uint32_t kRegionCands = 6;   // candidate table ranges timed
 static constexpr float kRegionNear = 1.06f;   // within this factor of a mode: in that mode
 static constexpr uint32_t kRegionBatch = 8;   // pieces allocated between classifications
 
-static bool placeable(const Ctx& c, const HbmDecision& d) {
-    return c.place_cands >= 2 && !c.ws_static && !c.graph_on && d.nbuf == 2 && d.h.nwin == 1 &&
+// tables_out: the launch returns the tables, which it copies as one range.
+static bool placeable(const Ctx& c, const HbmDecision& d, bool tables_out) {
+    return c.place_cands >= 2 && !c.ws_static && !c.graph_on && !tables_out && d.nbuf == 2 && d.h.nwin == 1 &&
            d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer &&
            c.budget >= 2 * d.h.off_tables + 3 * (d.h.buffer - d.h.off_tables);
 }
@@ -1790,10 +1804,13 @@ void drop_pool(RegionPool& p, const std::vector<hipMemGenericAllocationHandle_t>
 static int place_regions(Ctx& c, const HbmDecision& d, bool* placed) {
     *placed = false;
     const size_t rest = d.h.off_tables, tabs = d.h.buffer - d.h.off_tables;
-    const size_t P = ((tabs + 2) / 3 + kRegionPieceAlign - 1) & ~(kRegionPieceAlign - 1);
+    // Each of a range's three pieces holds every third slot's table; pieces
+    // are at least kRegionMinPiece, so that the pair walks can sort them.
+    const size_t third = ((size_t(d.chunk) + 2) / 3) * d.h.table_bytes;
+    const size_t P = std::max(kRegionMinPiece, (third + kRegionPieceAlign - 1) & ~(kRegionPieceAlign - 1));
     const size_t room = c.budget > 2 * rest ? c.budget - 2 * rest : 0;
     const size_t maxp = std::min<size_t>(kRegionMaxPieces, room / P);
-    if (P < kRegionMinPiece || maxp < 6) return CK_OK;
+    if (maxp < 6) return CK_OK;
     auto* sink = reinterpret_cast<unsigned long long*>(c.ws.base);  // the first pass's counters, cleared per pass
     RegionPool rp;
     std::string err;
@@ -1828,9 +1845,10 @@ static int place_regions(Ctx& c, const HbmDecision& d, bool* placed) {
             const hipError_t e = timed_twice(
                 c.stream, c.ev_chain[0][0], c.ev_chain[0][1],
                 [&]() {
+                    char* t0 = static_cast<char*>(test);
                     hipLaunchKernelGGL(k_place_walk, dim3((d.chunk + 255) / 256), dim3(256), 0, c.stream,
-                                       static_cast<uint64_t*>(test), pitch, static_cast<uint32_t>(pitch), d.chunk,
-                                       kPlaceWalkSteps, sink);
+                                       TabRef{{t0, t0 + P, t0 + 2 * P}, 3}, pitch, static_cast<uint32_t>(pitch),
+                                       d.chunk, kPlaceWalkSteps, sink);
                 },
                 &cms[i]);
             if (e != hipSuccess) {
@@ -1926,8 +1944,8 @@ static int place_buffers(Ctx& c, const HbmDecision& d) {
             c.stream, c.ev_chain[0][0], c.ev_chain[0][1],
             [&]() {
                 hipLaunchKernelGGL(k_place_walk, dim3((lanes + 255) / 256), dim3(256), 0, c.stream,
-                                   static_cast<uint64_t*>(cand[i]), pitch, static_cast<uint32_t>(pitch), lanes,
-                                   kPlaceWalkSteps, sink);
+                                   TabRef{{static_cast<char*>(cand[i]), nullptr, nullptr}, 1}, pitch,
+                                   static_cast<uint32_t>(pitch), lanes, kPlaceWalkSteps, sink);
             },
             &ms[i]);
         if (e != hipSuccess) {
@@ -1975,7 +1993,7 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         rc = decide_hbm(c, p, count, out.tables != nullptr, graph_wanted, !out.map && !out.walks && !out.rnds, d,
                         lane);
         if (rc) return rc;
-        placed = placeable(c, d);
+        placed = placeable(c, d, out.tables != nullptr);
         rc = placed ? place_buffers(c, d) : ensure_ws(c, d.nbuf * d.h.buffer);
         if (rc != CK_ENOMEM || attempt == 8 || d.chunk == 1) break;
         c.budget /= 2;
@@ -1995,7 +2013,15 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         std::memcpy(a.bk.w, rk, sizeof a.bk.w);
         a.map = base + h.off_map;
         a.map_stride = h.map_stride;
-        a.tables = placed ? static_cast<char*>(c.tabs[b].base) : base + h.off_tables;
+        if (!placed) {
+            a.tables = TabRef{{base + h.off_tables, nullptr, nullptr}, 1};
+        } else if (c.tabs[b].pieces.size() == 3) {  // by region: every third slot in each piece
+            char* t0 = static_cast<char*>(c.tabs[b].base);
+            const size_t P = c.tabs[b].piece;
+            a.tables = TabRef{{t0, t0 + P, t0 + 2 * P}, 3};
+        } else {
+            a.tables = TabRef{{static_cast<char*>(c.tabs[b].base), nullptr, nullptr}, 1};
+        }
         a.table_bytes = h.table_bytes;
         a.walks = reinterpret_cast<uint64_t*>(base + h.off_walk);
         a.rnds = reinterpret_cast<uint64_t*>(base + h.off_rnd);
@@ -2080,7 +2106,8 @@ static int run_hbm(Ctx& c, const ck_params& p, const uint32_t rk[44], uint32_t f
         const size_t done = size_t(i) * chunk;
         if (out.probes) CKH(hipMemcpy(out.probes + done, a.probes, size_t(a.count) * 4, hipMemcpyDeviceToHost));
         if (out.tables &&
-            (r = copy_rows(out.tables + done * p.buckets, a.tables, h.table_bytes, p.buckets, a.count, h.table16, true)))
+            (r = copy_rows(out.tables + done * p.buckets, a.tables.base[0], h.table_bytes, p.buckets, a.count, h.table16,
+                           true)))
             return r;
         const bool map64 = h.desc || h.stream64;
         if (out.map && !map64 &&

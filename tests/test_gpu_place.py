@@ -1,7 +1,8 @@
 """Table placement: the tables of a placed plan's pass buffers are held apart
 from them, spread over the device's memory regions (pieces sorted into regions
-by pair walks), or, where the pieces would be too small, the two fastest of
-candidate allocations.  Results must not depend on it."""
+by pair walks, every third slot's table in each), or, where the memory budget
+holds too few pieces, the two fastest of candidate allocations.  Results must
+not depend on it."""
 import pytest
 
 from paths import load_product
@@ -63,20 +64,36 @@ def test_region_placement_follows_the_table_size(layout):
         assert reports[0] != reports[1]
 
 
-def test_search_where_pieces_would_be_small():
-    """Tables of 1.6 GB per pass buffer (n = 4096) would split into pieces under
-    1 GiB: the library searches whole candidate allocations and keeps the two
-    fastest."""
-    p = ck.params("bgh25", 4096, 16, kick=500)
+@pytest.mark.parametrize("pnum", [1024, 4096])
+def test_region_placement_of_small_tables(pnum):
+    """Tables of 0.4 and 1.6 GB per pass buffer: each of a range's three 1 GiB
+    pieces holds the tables of every third slot, from its start; results equal
+    the unplaced context's."""
+    p = ck.params("bgh25", pnum, 16, kick=500)
     rk = ck.batch_key(3, 0)
     with ck.Context(0) as a, ck.Context(0) as b:
         a.set_place_search(8)
-        a.set_layout(ck.LAYOUT_RESIDUAL)
+        fa, sa = a.launch(p, rk, 0, 1 << 17)
+        _check_regions(a)
+        fb, sb = b.launch(p, rk, 0, 1 << 17)
+        assert list(fa) == list(fb) and sa.probes == sb.probes
+
+
+def test_search_where_the_budget_holds_few_pieces():
+    """A memory budget of 10 GiB holds five 1 GiB pieces beside the pass
+    buffers at n = 4096 (2.2 GB of maps and 1.6 GB of tables each), too few to
+    sort: the library searches three whole candidate table allocations instead
+    and keeps the two fastest."""
+    p = ck.params("bgh25", 4096, 16, kick=500)
+    rk = ck.batch_key(3, 0)
+    with ck.Context(0) as a, ck.Context(0) as b:
+        a.set_placement(layout=ck.LAYOUT_RESIDUAL, budget_bytes=10 << 30)
+        a.set_place_search(8)
         b.set_layout(ck.LAYOUT_RESIDUAL)
         fa, sa = a.launch(p, rk, 0, 1 << 17)
         ms, kept = a.place_report()
         assert a.place_regions()[0] == []
-        assert 3 <= len(ms) <= 8 and all(x > 0 for x in ms)
+        assert 2 <= len(ms) <= 3 and all(x > 0 for x in ms)
         order = sorted(range(len(ms)), key=lambda i: ms[i])
         assert (ms[kept[0]], ms[kept[1]]) == (ms[order[0]], ms[order[1]])
         fb, sb = b.launch(p, rk, 0, 1 << 17)
@@ -84,7 +101,7 @@ def test_search_where_pieces_would_be_small():
 
 
 def test_place_search_skips_small_and_graph_plans():
-    small = ck.params("bgh25", 1024, 16, kick=500)  # 0.4 GB pass buffers
+    small = ck.params("bgh25", 256, 16, kick=500)  # 0.25 GB pass buffers
     with ck.Context(0) as a:
         a.set_place_search(16)
         a.launch(small, ck.batch_key(0, 0), 0, 1 << 17)
