@@ -510,6 +510,7 @@ def config1_leg(ck):
     p = ck.params("bgh25", 1024, 16, kick=500)
     op = ock.parameters("bgh25", 1024, 16, kick=500)
     with ck.Context(0) as ctx:
+        ctx.set_place_search(32)  # tables spread over the memory regions, placed by the untimed launch
         ctx.launch(p, ck.batch_key(0, 1), 0, n)  # untimed: allocates the workspace of this trial count
         t0 = time.perf_counter()
         f, st = ctx.launch(p, ck.batch_key(0, 0), 0, n)
@@ -540,6 +541,7 @@ def tiers_leg(ck):
         for tier, tname in ((ck.TIER_HBM, "hbm"), (ck.TIER_LANE, "lane")):
             with ck.Context(0) as ctx:
                 ctx.set_tier(tier)
+                ctx.set_place_search(32)  # HBM tier: tables spread over the memory regions where placeable
                 if tier == ck.TIER_LANE:
                     row["lane_lds_per_block"] = ctx.plan(p, 1 << 16).chain_lds_bytes
                 ctx.set_chunk(1 << 16)

@@ -1585,21 +1585,25 @@ static int decide_hbm(const Ctx& c, const ck_params& p, uint32_t count, bool tab
 // runs fast (73-79 ms): neither the contiguous-allocation flag nor any
 // allocation size changes that.
 //
-// A double-buffered plan with large pass buffers (placeable) therefore holds
-// the tables of each pass buffer apart from it.  place_regions allocates the
-// tables in pieces of a third (virtual-memory API), as many as the budget
-// holds, sorts them into regions by pair walks (k_place_pair) and maps one
-// piece from each of three regions (2:1 over two where only two show) into
-// each pass buffer's table range.  Where pieces would be under kRegionMinPiece
-// (the pair walk needs 768 MiB per piece to stay out of the caches) or fewer
-// than two regions show, place_buffers falls back to a search: up to
-// place_cands candidate table allocations, k_place_walk (the chains' access
-// pattern) over each, the two fastest kept.  The rest of both pass buffers
-// (candidate maps, per-slot words) shares one allocation; its placement did
-// not change the chain.  Placement reruns only when a plan needs tables of
-// another size, so it is for contexts that run many passes of one shape
-// (ck_ctx_set_place_search turns it on; the CLI and the headline bench do).
-// Results do not depend on it: every pass computes the same trials anywhere.
+// A double-buffered plan (placeable) therefore holds the tables of each pass
+// buffer apart from it.  place_regions allocates pieces of a third of a pass's
+// tables (virtual-memory API; kRegionMinPiece at least, since the pair walk
+// needs 768 MiB per piece to stay out of the caches -- small tables use the
+// start of each piece), as many as the budget holds, sorts them into regions
+// by pair walks (k_place_pair) and gives each pass buffer one piece from each
+// of three regions (2:1 over two where only two show), every third slot's
+// table in each piece (TabRef n = 3), so that the lanes of a wave and the
+// slots still running at a pass's end spread over the regions too.  Small
+// tables gain as much as large ones (n = 1024 to 8192: 15-23 % more trials/s).
+// Where fewer than two regions show or the budget holds fewer than six
+// pieces, place_buffers falls back to a search: up to place_cands candidate
+// table allocations, k_place_walk (the chains' access pattern) over each, the
+// two fastest kept.  The rest of both pass buffers (candidate maps, per-slot
+// words) shares one allocation; spreading it too changed nothing.  Placement
+// reruns only when a plan needs tables of another size, so it is for contexts
+// that run many passes of one shape (ck_ctx_set_place_search turns it on; the
+// CLI and the headline bench do).  Results do not depend on it: every pass
+// computes the same trials anywhere.
 // Placed plans: passes of at least kPlaceMinChains chains (a walk over fewer
 // lanes does not load the random-access path) in buffers of at least 256 MiB.
 static constexpr size_t kPlaceMinBuffer = size_t(256) << 20;
@@ -1617,7 +1621,7 @@ static constexpr uint32_t kRegionBatch = 8;   // pieces allocated between classi
 // tables_out: the launch returns the tables, which it copies as one range.
 static bool placeable(const Ctx& c, const HbmDecision& d, bool tables_out) {
     return c.place_cands >= 2 && !c.ws_static && !c.graph_on && !tables_out && d.nbuf == 2 && d.h.nwin == 1 &&
-           d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer &&
+           !d.h.lane && d.h.table_bytes > 0 && d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer &&
            c.budget >= 2 * d.h.off_tables + 3 * (d.h.buffer - d.h.off_tables);
 }
 
@@ -1793,14 +1797,14 @@ void drop_pool(RegionPool& p, const std::vector<hipMemGenericAllocationHandle_t>
     p = RegionPool{};
 }
 
-// Region placement of a plan's tables (see above): pieces of a third of a
-// pass buffer's tables, up to six candidate ranges of one piece per region
-// (or, two regions seen, one piece of piece 0's region and two of the other,
-// from either end of their allocation order), each timed with k_place_walk at
-// the plan's own geometry; the two fastest become the table ranges.  *placed
-// stays false, with nothing allocated beyond ws, when it does not apply:
-// pieces too small, no virtual-memory support, fewer than two regions seen,
-// or too few pieces in them.
+// Region placement of a plan's tables (see above): pieces holding every
+// third slot's table (1 GiB at least), up to six candidate ranges of one piece
+// per region (or, two regions seen, one piece of piece 0's region and two of
+// the other, from either end of their allocation order), each timed with
+// k_place_walk at the plan's own geometry; the two fastest become the table
+// ranges.  *placed stays false, with nothing allocated beyond ws, when it does
+// not apply: too few pieces in the budget, no virtual-memory support, fewer
+// than two regions seen, or too few pieces in them.
 static int place_regions(Ctx& c, const HbmDecision& d, bool* placed) {
     *placed = false;
     const size_t rest = d.h.off_tables, tabs = d.h.buffer - d.h.off_tables;
