@@ -118,3 +118,22 @@ def test_place_search_rejects_one_candidate():
     with ck.Context(0) as a:
         with pytest.raises(ck.CkError):
             a.set_place_search(1)
+
+
+@pytest.mark.parametrize("layout", [ck.LAYOUT_RESIDUAL, ck.LAYOUT_DESC_PLAIN, ck.LAYOUT_MAP])
+def test_region_placed_tables_hold_the_same_entries(layout):
+    """Per-trial table digests and probe counts of a region-placed context
+    (every third slot's table in each piece) equal the unplaced context's, for
+    residual, descriptor and bucket-map entries (n = 2048)."""
+    p = ck.params("bgh25", 2048, 16, kick=500)
+    rk = ck.batch_key(5, 0)
+    got = []
+    for place in (8, 0):
+        with ck.Context(0) as c:
+            c.set_layout(layout)
+            c.set_place_search(place)
+            c.launch(p, rk, 0, 1 << 17)  # places the tables (the digests launch runs in them)
+            assert (len(c.place_regions()[0]) >= 6) == bool(place)
+            got.append(c.launch_digest(p, rk, 65536, 1 << 17))
+    (fa, pa, da, _), (fb, pb, db, _) = got
+    assert list(fa) == list(fb) and (pa == pb).all() and (da == db).all()
