@@ -197,7 +197,7 @@ def cpu_reference(cores: int, gpu_failures_prefix):
             "gpu_failures_equal": failed == gpu_failures_prefix}
 
 
-def random_access_ceiling(probes_per_s: float, requests_per_probe):
+def random_access_ceiling(probes_per_s: float, requests_per_probe, stream_requests_per_probe: float = 0.0):
     """Runs lib/ck_randbench: the chain's access pattern (each lane a dependent
     random read + write of one 8-byte slot of its own 192 KB table, with one
     XOR and one 32-bit multiply-high between a load and the next address)
@@ -205,7 +205,10 @@ def random_access_ceiling(probes_per_s: float, requests_per_probe):
     in one range spread over the device's memory regions (three pieces, as the
     library places the chains' tables).  The ceiling is the best random-read
     request rate measured, with one walk per lane or four independent walks per
-    lane (latency out of the way)."""
+    lane (latency out of the way).  The walks' requests are all random lines;
+    frac sets against the ceiling the chain's random requests, its total less
+    `stream_requests_per_probe` (the candidate stream, read in order, one
+    64-byte line per eight inserted items)."""
     import re
     import subprocess
     from paths import PKG
@@ -240,6 +243,7 @@ def random_access_ceiling(probes_per_s: float, requests_per_probe):
     rw4 = max(v.get("read_write_w4", 0) for v in rows_all)
     rw78 = max(v.get("read_write_7of8", 0) for v in rows_all)
     chain_req = probes_per_s * requests_per_probe if requests_per_probe else None
+    chain_rand = probes_per_s * (requests_per_probe - stream_requests_per_probe) if requests_per_probe else None
     # The most DRAM requests per second any of the walks got through.
     ceiling_req = max(ro1 * RB_READ_ONLY_REQ_PER_STEP, ro4 * RB_READ_ONLY_W4_REQ_PER_STEP,
                       rw * RB_READ_WRITE_REQ_PER_STEP, rw4 * RB_READ_WRITE_W4_REQ_PER_STEP,
@@ -260,15 +264,19 @@ def random_access_ceiling(probes_per_s: float, requests_per_probe):
             # second against the most any of the walks got through (read-only or
             # read + write, one or four walks per lane, any buffer or range).
             "chain_dram_requests_per_s": chain_req,
+            "chain_stream_requests_per_probe": stream_requests_per_probe,
+            "chain_random_requests_per_s": chain_rand,
             "ceiling_requests_per_s": ceiling_req,
             "read_only_requests_per_s": max(ro1 * RB_READ_ONLY_REQ_PER_STEP, ro4 * RB_READ_ONLY_W4_REQ_PER_STEP),
             "read_write_requests_per_s": rw * RB_READ_WRITE_REQ_PER_STEP,
-            "frac": chain_req / ceiling_req if chain_req else None,
+            "frac": chain_rand / ceiling_req if chain_rand else None,
+            "frac_all_requests": chain_req / ceiling_req if chain_req else None,
             "note": f"requests per probe / per step are profile-derived (TCC_EA0_RDREQ + WRREQ): chain "
                     f"{requests_per_probe:.4f}, read + write step 1.999 (one or four walks per lane), seven in "
                     f"eight written back 1.804, read-only 0.998 (one walk per lane) or 0.9636 (four). The walks' "
-                    f"requests are all random lines; about 0.06 of the chain's per probe are its sequential "
-                    f"candidate stream, so frac can pass 1"}
+                    f"requests are all random lines; frac counts the chain's random requests only (its total less "
+                    f"{stream_requests_per_probe:.4f} per probe for the candidate stream, read in order); "
+                    f"frac_all_requests counts them all"}
 
 
 def load_sweep():
@@ -832,8 +840,11 @@ def main():
         },
         "device_ms": {"build": agg["build_ms"], "chain": agg["chain_ms"]},
     }
+    # The candidate stream: item_bytes per inserted item, read in order, one
+    # 64-byte line per 64 / item_bytes items.
+    stream_req = (p.pnum / probes_per_trial) * prof["item_bytes"] / 64 if prof and probes_per_trial else 0.0
     rec["roofline"]["random_access_ceiling"] = random_access_ceiling(probes_per_launch / chain_avg_s,
-                                                                     prof["requests"] if prof else None)
+                                                                     prof["requests"] if prof else None, stream_req)
     # The table placement of this context: by region (pieces per region, the
     # regions of each pass buffer's three pieces, walk ms of each candidate
     # table range, the two kept) or by search (walk ms per candidate, the two
