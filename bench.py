@@ -572,12 +572,13 @@ def tiers_leg(ck):
     return out, ok
 
 
-def graph_leg(ck, p, steps: int, trials: int):
+def graph_leg(ck, p, steps: int, trials: int, place: int):
     """Graph mode against kernel-by-kernel launches on three launch sizes:
     the headline steps (2^20 trials), the n = 1024 point (1e5 trials per
     launch) and small n = 256 launches of 1,024 trials, where host launch work
-    weighs most.  Each form runs in its own context after one untimed launch
-    (which builds the graph); the failure lists of every launch must agree."""
+    weighs most.  Each form runs in its own context, with the headline's table
+    placement, after one untimed launch (which places the tables and builds
+    the graph); the failure lists of every launch must agree."""
     cases = [("headline n=16384, 2^20 trials per launch", p, trials, min(steps, 6)),
              ("n=1024 m=1536 kick 500, 1e5 trials per launch", ck.params("bgh25", 1024, 16, kick=500), 100000, 20),
              ("n=256 m=384 kick 500, 1024 trials per launch", ck.params("bgh25", 256, 16, kick=500), 1024, 200)]
@@ -587,6 +588,7 @@ def graph_leg(ck, p, steps: int, trials: int):
         for mode in ("kernels", "graph"):
             with ck.Context(0) as ctx:
                 ctx.set_graph(mode == "graph")
+                ctx.set_place_search(place)
                 ctx.launch(shp, ck.batch_key(SEED, 0), 0, count)
                 got = []
                 t0 = time.perf_counter()
@@ -892,7 +894,7 @@ def main():
     if not args.no_graph:
         # Last: a graph-mode context leaves its workspace allocated until the
         # process ends, which would shrink the passes of any later leg.
-        rec["graph"] = graph_leg(ck, p, args.steps, t)
+        rec["graph"] = graph_leg(ck, p, args.steps, t, args.place_search)
         ok = ok and all(g["results_equal"] for g in rec["graph"])
     print(json.dumps(rec), flush=True)
     return 0 if ok else 1

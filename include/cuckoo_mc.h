@@ -134,7 +134,7 @@ int ck_ctx_set_chunk(ck_ctx* ctx, uint32_t trials);
  * reference keeps no per-trial candidate stream). */
 int ck_ctx_set_window(ck_ctx* ctx, uint32_t items);
 /* Table placement for double-buffered HBM-tier passes of 16,384 or more
- * trials whose pass buffers are 256 MiB or larger (outside graph mode,
+ * trials whose pass buffers are 256 MiB or larger (in graph mode too; not for
  * ck_launch_static and launches that return their tables); 0, the default,
  * turns it off.  The cuckoo tables of the two pass buffers are then held apart
  * from them.  MI355X memory falls into three regions of about 96 GB, and
@@ -180,7 +180,9 @@ int ck_ctx_set_layout(ck_ctx* ctx, uint32_t layout);
  * an instantiated graph's kernel nodes pointed into after hipFree, until the
  * process ends; the library therefore keeps such workspaces itself and hands
  * them to the next context on the device, so a process holds one of them per
- * device, not one per graph-mode context. */
+ * device, not one per graph-mode context.  Placed tables (virtual-memory
+ * pieces) are not affected: they are unmapped and released as outside graph
+ * mode. */
 int ck_ctx_set_graph(ck_ctx* ctx, uint32_t on);
 
 /* How bucket evaluations get their Feistel round function. */

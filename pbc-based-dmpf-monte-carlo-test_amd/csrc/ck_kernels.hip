@@ -1014,7 +1014,10 @@ static int free_tables(Ctx& c, TableRange& t) {
             retire_ws(c.device, Workspace{t.base, t.bytes});
         else
             CKH(hipFree(t.base));
-    } else {  // never in a graph (placed plans are not captured); unmapped once idle
+    } else {
+        // Virtual-memory pieces: unmapped and released once idle, graph use or
+        // not (the runtime returns them to the device, unlike hipMalloc memory
+        // that graph nodes pointed into).  Graphs pointing here are gone by now.
         CKH(hipDeviceSynchronize());
         for (size_t i = 0; i < t.pieces.size(); ++i) CKH(hipMemUnmap(static_cast<char*>(t.base) + i * t.piece, t.piece));
         for (hipMemGenericAllocationHandle_t h : t.pieces) CKH(hipMemRelease(h));
@@ -1620,7 +1623,7 @@ static constexpr uint32_t kRegionBatch = 8;   // pieces allocated between classi
 
 // tables_out: the launch returns the tables, which it copies as one range.
 static bool placeable(const Ctx& c, const HbmDecision& d, bool tables_out) {
-    return c.place_cands >= 2 && !c.ws_static && !c.graph_on && !tables_out && d.nbuf == 2 && d.h.nwin == 1 &&
+    return c.place_cands >= 2 && !c.ws_static && !tables_out && d.nbuf == 2 && d.h.nwin == 1 &&
            !d.h.lane && d.h.table_bytes > 0 && d.chunk >= kPlaceMinChains && d.h.buffer >= kPlaceMinBuffer &&
            c.budget >= 2 * d.h.off_tables + 3 * (d.h.buffer - d.h.off_tables);
 }
@@ -1905,14 +1908,21 @@ static int place_buffers(Ctx& c, const HbmDecision& d) {
     if (c.placed_for == tabs && c.ws.bytes >= 2 * rest && c.tabs[0].bytes >= tabs && c.tabs[1].bytes >= tabs)
         return CK_OK;
     if (int rc = release_ws(c)) return rc;
-    hipError_t e = hipMalloc(&c.ws.base, 2 * rest);
-    if (e == hipErrorOutOfMemory) {
-        (void)hipGetLastError();
-        c.ws.base = nullptr;
-        return fail(CK_ENOMEM, "device memory: cannot allocate the pass buffers");
+    hipError_t e = hipSuccess;
+    Workspace reuse;
+    if (take_retired(c.device, 2 * rest, reuse)) {  // one a graph-mode context left
+        c.ws = reuse;
+        c.ws_graphed = true;
+    } else {
+        e = hipMalloc(&c.ws.base, 2 * rest);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            c.ws.base = nullptr;
+            return fail(CK_ENOMEM, "device memory: cannot allocate the pass buffers");
+        }
+        if (e != hipSuccess) return fail(CK_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+        c.ws.bytes = 2 * rest;
     }
-    if (e != hipSuccess) return fail(CK_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
-    c.ws.bytes = 2 * rest;
     bool placed = false;
     if (int rc = place_regions(c, d, &placed)) return rc;
     if (placed) return CK_OK;

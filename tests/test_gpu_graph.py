@@ -106,22 +106,27 @@ def device_free_bytes():
     return free.value
 
 
-def test_graph_contexts_do_not_accumulate_memory():
+@pytest.mark.parametrize("place", [0, 16], ids=["unplaced", "placed"])
+def test_graph_contexts_do_not_accumulate_memory(place):
     """The runtime keeps a workspace that graph nodes pointed into after it is
     freed; the library keeps such workspaces for reuse instead, so a sequence
     of graph-mode contexts (and an eager one after them) holds one workspace,
-    not one per context.  Results stay those of kernel-by-kernel launches."""
+    not one per context.  Placed tables (virtual-memory pieces) go back to the
+    device.  Results stay those of kernel-by-kernel launches."""
     p = ck.params("bgh25", 4096, 16, kick=500)
+    count = 1 << 17  # two passes: double-buffered, so placeable
     with ck.Context(0) as ctx:
-        want, _ = ctx.launch(p, ck.batch_key(3, 0), 0, 1 << 16)
+        want, _ = ctx.launch(p, ck.batch_key(3, 0), 0, count)
     free_after = []
     for graph in (True, True, True, False):
         with ck.Context(0) as ctx:
+            ctx.set_place_search(place)
             ctx.set_graph(graph)
             for _ in range(2):
-                f, st = ctx.launch(p, ck.batch_key(3, 0), 0, 1 << 16)
+                f, st = ctx.launch(p, ck.batch_key(3, 0), 0, count)
                 assert list(f) == list(want)
             assert st.graph == (1 if graph else 0)
+            assert (ctx.place_report()[1] != (-1, -1)) == (place > 0)
         free_after.append(device_free_bytes())
     slack = 1 << 30
     assert all(f >= free_after[0] - slack for f in free_after[1:]), [f >> 30 for f in free_after]

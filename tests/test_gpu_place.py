@@ -100,18 +100,30 @@ def test_search_where_the_budget_holds_few_pieces():
         assert list(fa) == list(fb) and sa.probes == sb.probes
 
 
-def test_place_search_skips_small_and_graph_plans():
+def test_place_search_skips_small_plans():
     small = ck.params("bgh25", 256, 16, kick=500)  # 0.25 GB pass buffers
     with ck.Context(0) as a:
         a.set_place_search(16)
         a.launch(small, ck.batch_key(0, 0), 0, 1 << 17)
         assert a.place_report() == ([], (-1, -1))
-    head = ck.params("bgh25", 16384, 16, kick=500)
-    with ck.Context(0) as g:
+
+
+def test_graph_replays_run_in_placed_tables():
+    """Graph mode places the tables like kernel-by-kernel launches do; the
+    graph's replays (new batch keys and offsets) give the unplaced eager
+    context's failing trials and probe totals."""
+    p = ck.params("bgh25", 4096, 16, kick=500)
+    with ck.Context(0) as g, ck.Context(0) as e:
         g.set_place_search(16)
         g.set_graph(True)
-        g.launch(head, ck.batch_key(1, 0), 0, 1 << 17)
-        assert g.place_report()[1] == (-1, -1)
+        for rep, (batch, first) in enumerate([(0, 0), (1, 777), (2, 5)]):
+            rk = ck.batch_key(6, batch)
+            f, st = g.launch(p, rk, first, 1 << 17)
+            assert st.graph == (2 if rep == 0 else 1)
+            assert g.place_report()[1] != (-1, -1)
+            ef, est = e.launch(p, rk, first, 1 << 17)
+            assert list(f) == list(ef), (batch, first)
+            assert st.probes == est.probes and st.failed == est.failed
 
 
 def test_place_search_rejects_one_candidate():
